@@ -1,0 +1,141 @@
+/*
+ * mops.h - C ABI of the gfx950 (MI355X) dense FP64 path of matops.
+ *
+ * Everything is column-major FP64 with a leading dimension ("step") equal to
+ * the parent matrix's row count, exactly as calgo's views (calgo/cmops.h:47-53).
+ * Flag bits are calgo's (calgo/cmops.h:13-24, calgo/cmops.go:19-31).
+ *
+ * Two families of entry points:
+ *  - host-memory drop-ins: same parameter lists as the calgo C kernels the
+ *    Go package binds through cgo (calgo/cmops.go). They copy the touched
+ *    blocks to the GPU, compute, copy results back and return when done.
+ *  - device-resident twins (mops_d_*): device pointers and a HIP stream
+ *    (hipStream_t passed as void*, NULL = the legacy default stream). They
+ *    are asynchronous unless stated otherwise.
+ *
+ * Return value: 0 on success; a negative value -k means argument k (1-based)
+ * is invalid; other negative values are runtime failures whose text is in
+ * mops_last_error(). Positive values are LAPACK-style info codes documented
+ * per routine. Every call is re-entrant: state (stream, workspace, error
+ * text) is kept per host thread.
+ */
+#ifndef MOPS_H
+#define MOPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Column-major view, pointer at element (0,0); same layout as calgo's
+ * mdata_t (calgo/cmops.h:50-53) so a cgo caller can pass its struct as is. */
+typedef struct mops_mdata {
+  double *md;
+  int step;
+} mops_mdata_t;
+
+enum {
+  MOPS_NOTRANS = 0,
+  MOPS_TRANSA = 0x1,
+  MOPS_TRANSB = 0x2,
+  MOPS_LOWER = 0x4,
+  MOPS_UPPER = 0x8,
+  MOPS_LEFT = 0x10,
+  MOPS_RIGHT = 0x20,
+  MOPS_UNIT = 0x40,
+  MOPS_TRANS = 0x80,
+  MOPS_NOSCALE = 0x800
+};
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Text of the calling thread's last failure ("" if none). */
+const char *mops_last_error(void);
+/* Number of visible GPUs (0 when no GPU / no driver). */
+int mops_device_count(void);
+/* Frees the calling thread's device workspace and stream. */
+void mops_thread_release(void);
+
+/* Launch timing for measurement (no counterpart in the reference). When
+ * enabled, the trailing updates of the factorizations are bracketed by HIP
+ * events on their stream; collect sums the elapsed time (ms) and the
+ * algorithmic flops of one class: 0 = SYRK trailing update of the Cholesky,
+ * 1 = GEMM trailing update of the LU, 2 = LU panel. Enabling again clears. */
+void mops_trace_enable(int on);
+int mops_trace_collect(int cls, int *count, double *ms, double *flops);
+
+/* ---- GEMM: C = beta*C + alpha*op(A)*op(B) -------------------------------- */
+
+/* Replaces dmult_mm_blocked3 (calgo/cmops.h:133-138, calgo/dmmat_gemm.c:364),
+ * bound by calgo.DMult (calgo/cmops.go:68-91) under matops.Mult
+ * (mmult.go:198-256). Updates the block C[R:E, S:L] with panel length P;
+ * rows R:E of op(A) and columns S:L of op(B) are used. beta == 0 writes
+ * zeros first (NaN in C is cleared), beta == 1 leaves C as is. vlen/NB/MB
+ * are the CPU blocking hints; they are accepted and ignored. */
+int mops_dmult_mm_blocked3(mops_mdata_t *C, const mops_mdata_t *A, const mops_mdata_t *B,
+                           double alpha, double beta, int flags, int P, int S, int L, int R,
+                           int E, int vlen, int NB, int MB);
+
+/* Device-resident GEMM. C is M x N, op(A) is M x P, op(B) is P x N; flags
+ * TRANSA / TRANSB select op(). */
+int mops_d_dgemm(int flags, int M, int N, int P, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc, void *stream);
+
+/* ---- TRSM: B = alpha*op(A)^-1*B (LEFT) or alpha*B*op(A)^-1 (RIGHT) ------- */
+
+/* Replaces dmmat_solve_blk (calgo/cmops.h:207-209, calgo/dmmat_solve.c:864),
+ * bound by calgo.DSolveBlk (calgo/cmops.go:195-214) under matops.SolveTrm
+ * (mmult.go:388-422). A is N x N; LEFT (no RIGHT bit) solves the N x (E-S)
+ * block of columns S:E of B, RIGHT the (E-S) x N block of rows S:E. UPPER
+ * selects the upper triangle (otherwise lower), TRANSA transposes A, UNIT
+ * never reads the diagonal. NB is a CPU hint, ignored. */
+int mops_dmmat_solve_blk(mops_mdata_t *B, const mops_mdata_t *A, double alpha, int flags, int N,
+                         int S, int E, int NB);
+
+/* Device-resident TRSM: B is N x nrhs (LEFT) or nrhs x N (RIGHT). */
+int mops_d_dtrsm(int flags, int N, int nrhs, double alpha, const double *A, int lda, double *B,
+                 int ldb, void *stream);
+
+/* ---- SYRK: C = beta*C + alpha*A*A^T (or A^T*A with TRANSA), one triangle - */
+
+/* Replaces dmmat_rank_blk (calgo/cmops.h:212-214, calgo/dmmat_rank.c:62),
+ * bound by calgo.DSymmRankBlk (calgo/cmops.go:219-240) under
+ * matops.RankUpdateSym (mmult.go:426-445). Updates the LOWER (else UPPER)
+ * triangle of the block C[S:E, S:E] with rows S:E of A (columns with
+ * TRANSA), inner length P. beta scales the whole triangle (BLAS semantics;
+ * the reference scales only its diagonal blocks, a difference that only
+ * shows for beta != 1, which no caller on the factorization path uses). */
+int mops_dmmat_rank_blk(mops_mdata_t *C, const mops_mdata_t *A, double alpha, double beta,
+                        int flags, int P, int S, int E, int vlen, int NB);
+
+/* Device-resident SYRK: C is N x N, A is N x P (P x N with TRANSA). */
+int mops_d_dsyrk(int flags, int N, int P, double alpha, const double *A, int lda, double beta,
+                 double *C, int ldc, void *stream);
+
+/* ---- Cholesky ------------------------------------------------------------ */
+
+/* Replaces DecomposeCHOL / blockedCHOL / unblockedCHOL (chol.go:17-127):
+ * A = L*L^T (LOWER) or U^T*U (otherwise), in place, outer block nb (0 or
+ * nb > N: one block). The other triangle is not modified. Returns 0, or
+ * 1 + the 0-based global index of the first diagonal whose square root is
+ * NaN, with mops_last_error() = "illegal value at <index>: <value>" (the
+ * panic text of chol.go:35). Host memory, synchronous. */
+int mops_dpotrf(mops_mdata_t *A, int flags, int N, int nb);
+
+/* Replaces SolveCHOL (chol.go:147-158): B (N x nrhs) = A^-1 B using the
+ * factor in A. Host memory, synchronous. */
+int mops_dpotrs(mops_mdata_t *B, const mops_mdata_t *A, int flags, int N, int nrhs);
+
+/* Device-resident twins. info: device int, set to -1 when the factorization
+ * succeeded, else the 0-based index of the first NaN square root (may be
+ * NULL). */
+int mops_d_dpotrf(int flags, int N, int nb, double *A, int lda, int *info, void *stream);
+int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
+                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOPS_H */

@@ -1,0 +1,11 @@
+"""matops_amd: the dense FP64 path of matops (GEMM, TRSM, SYRK, Cholesky,
+LU with partial pivoting) on AMD MI355X (gfx950), behind a C ABI.
+
+The compute lives in libmops.so (hand-written HIP kernels); this package is
+the host-side mirror of the reference's Go API and the ctypes binding.
+"""
+from .matops import (  # noqa: F401
+    LEFT, LOWER, NONE, NOTRANS, RIGHT, TRANS, TRANSA, TRANSB, UNIT, UPPER,
+    BlockingParams, DecomposeCHOL, MatopsError, Mult, NumWorkers, RankUpdateSym,
+    SetPanicOnError, SolveCHOL, SolveTrm,
+)

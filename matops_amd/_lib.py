@@ -1,0 +1,77 @@
+"""ctypes binding of libmops.so (the C ABI declared in include/mops.h).
+
+The library is loaded from this package directory. There is no fallback:
+if libmops.so is missing, importing a function that needs it raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmops.so")
+
+
+class MData(ctypes.Structure):
+    """mops_mdata_t / calgo mdata_t: pointer to element (0,0) and leading dim."""
+    _fields_ = [("md", ctypes.c_void_p), ("step", ctypes.c_int)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PM = ctypes.POINTER(MData)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mops_last_error": (ctypes.c_char_p, []),
+    "mops_device_count": (_I, []),
+    "mops_thread_release": (None, []),
+    "mops_trace_enable": (None, [_I]),
+    "mops_trace_collect": (_I, [_I, _P, _P, _P]),
+    "mops_dmult_mm_blocked3": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I, _I, _I, _I]),
+    "mops_d_dgemm": (_I, [_I, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
+    "mops_dmmat_solve_blk": (_I, [_PM, _PM, _D, _I, _I, _I, _I, _I]),
+    "mops_d_dtrsm": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _P]),
+    "mops_dmmat_rank_blk": (_I, [_PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I]),
+    "mops_d_dsyrk": (_I, [_I, _I, _I, _D, _P, _I, _D, _P, _I, _P]),
+    "mops_dpotrf": (_I, [_PM, _I, _I, _I]),
+    "mops_dpotrs": (_I, [_PM, _PM, _I, _I, _I]),
+    "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),
+    "mops_d_dpotrs": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libmops.so once and declares every signature."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(
+                f"{LIBPATH} is not built; run `python -m matops_amd.build` first")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class MopsError(RuntimeError):
+    def __init__(self, func, code, msg):
+        super().__init__(f"{func} returned {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def check(func, code):
+    if code != 0:
+        msg = lib().mops_last_error()
+        raise MopsError(func, code, msg.decode() if msg else "")
+    return code
+
+
+def last_error():
+    msg = lib().mops_last_error()
+    return msg.decode() if msg else ""

@@ -1,0 +1,112 @@
+// C ABI: factorizations and solves (Cholesky here; LU, mixed precision and
+// the Cholesky adjoint in their own files).
+#include <cmath>
+#include <cstdio>
+
+#include "../../include/mops.h"
+#include "mops_solvers.hpp"
+
+using namespace mops;
+
+namespace {
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+int h2d(double* d, long ldd, const double* h, long ldh, int rows, int cols, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return ST_OK;
+  MOPS_CHECK_HIP(hipMemcpy2DAsync(d, ldd * sizeof(double), h, ldh * sizeof(double),
+                                  rows * sizeof(double), cols, hipMemcpyHostToDevice, s));
+  return ST_OK;
+}
+
+int d2h(double* h, long ldh, const double* d, long ldd, int rows, int cols, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return ST_OK;
+  MOPS_CHECK_HIP(hipMemcpy2DAsync(h, ldh * sizeof(double), d, ldd * sizeof(double),
+                                  rows * sizeof(double), cols, hipMemcpyDeviceToHost, s));
+  return ST_OK;
+}
+
+hipStream_t need_stream() {
+  hipStream_t s = host_stream();
+  if (!s) set_error("could not create a HIP stream");
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mops_d_dpotrf(int flags, int N, int nb, double* A, int lda, int* info, void* stream) {
+  clear_error();
+  if (N < 0) return -2;
+  if (nb < 0) return -3;
+  if (lda < (N > 1 ? N : 1)) return -5;
+  if (N == 0) return ST_OK;
+  MOPS_TRY(ws_reserve(potrf_ws_bytes(N, nb)));
+  return potrf_d(as_stream(stream), flags, N, nb, A, lda, info, nullptr);
+}
+
+int mops_d_dpotrs(int flags, int N, int nrhs, const double* A, int lda, double* B, int ldb,
+                  void* stream) {
+  clear_error();
+  if (N < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < (N > 1 ? N : 1)) return -5;
+  if (ldb < (N > 1 ? N : 1)) return -7;
+  if (N == 0 || nrhs == 0) return ST_OK;
+  MOPS_TRY(ws_reserve(potrs_ws_bytes(flags, N, nrhs)));
+  return potrs_d(as_stream(stream), flags, N, nrhs, A, lda, B, ldb);
+}
+
+int mops_dpotrf(mops_mdata_t* A, int flags, int N, int nb) {
+  clear_error();
+  if (!A || (N > 0 && !A->md)) return -1;
+  if (N < 0) return -3;
+  if (nb < 0) return -4;
+  if (N == 0) return ST_OK;
+  if (A->step < N) return -1;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)N * N;
+  MOPS_TRY(ws_reserve(na * sizeof(double) + potrf_ws_bytes(N, nb) + 2048));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  int* dinfo = (int*)ws_carve(sizeof(int));
+  double* dbad = (double*)ws_carve(sizeof(double));
+  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
+  MOPS_TRY(potrf_d(s, flags, N, nb, dA, N, dinfo, dbad));
+  MOPS_TRY(d2h(A->md, A->step, dA, N, N, N, s));
+  int info = -1;
+  double bad = 0.0;
+  MOPS_CHECK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  MOPS_CHECK_HIP(hipMemcpyAsync(&bad, dbad, sizeof(double), hipMemcpyDeviceToHost, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  if (info >= 0) {
+    // the panic text of unblockedCHOL (chol.go:35)
+    set_error("illegal value at %d: %e", info, bad);
+    return info + 1;
+  }
+  return ST_OK;
+}
+
+int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nrhs) {
+  clear_error();
+  if (!B || (N > 0 && nrhs > 0 && !B->md)) return -1;
+  if (!A || (N > 0 && !A->md)) return -2;
+  if (N < 0) return -4;
+  if (nrhs < 0) return -5;
+  if (N == 0 || nrhs == 0) return ST_OK;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + potrs_ws_bytes(flags, N, nrhs) + 1024));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  double* dB = (double*)ws_carve(nb * sizeof(double));
+  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
+  MOPS_TRY(h2d(dB, N, B->md, B->step, N, nrhs, s));
+  MOPS_TRY(potrs_d(s, flags, N, nrhs, dA, N, dB, N));
+  MOPS_TRY(d2h(B->md, B->step, dB, N, N, nrhs, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
+}  // extern "C"

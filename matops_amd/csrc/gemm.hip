@@ -1,0 +1,471 @@
+// Column-major GEMM on the gfx950 matrix cores.
+//
+// Replaces the blocked dot-product GEMM of calgo/dmmat_gemm.c:364-416
+// (dmult_mm_blocked3 -> _dblock_mult_gepp :309 -> _dblock_ddot_sse :14) and
+// its beta pre-scaling dscale_tile (calgo/vec_scale.c:114). The same kernel,
+// restricted to one triangle of C, is the SYRK of calgo/dmmat_rank.c:62-126.
+//
+// Design (one workgroup per BMxBN tile of C; 4 or 8 wave64s):
+//  * k is walked in BK-deep slabs. A slab of op(A) (BM x BK) and op(B)
+//    (BK x BN) is staged in LDS as [k][m] and [k][n] rows, double buffered;
+//    the next slab is fetched into registers while the current one feeds
+//    the matrix cores, so global latency hides behind MFMA work.
+//  * Each wave owns a WM x WN sub-tile made of 16x16 MFMA blocks
+//    (v_mfma_f64_16x16x4_f64, or v_mfma_f32_16x16x4_f32 for FP32). The
+//    instruction computes D = op(B)^T * op(A)^T, i.e. C transposed, so lane
+//    l holds C row l%16 and the 16 lanes of a group write one contiguous
+//    run of a column: stores to column-major C coalesce.
+//  * Workgroup ids are remapped so that each of the 8 XCDs receives a
+//    contiguous band of tiles (its own L2 then serves neighbouring tiles),
+//    and tiles are grouped GROUP_M tile-rows at a time for reuse of op(B).
+#include "mops_kernels.hpp"
+
+#include <cmath>
+
+namespace mops {
+namespace {
+
+template <typename T>
+struct Mfma;
+// Operand layout (both types): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] of the 16x4 and 4x16 operands. Result layout: register r of
+// lane l holds D[drow(l/16, r)][l%16]; the 64-bit result of the FP64
+// instruction interleaves rows (row = l/16 + 4r), the FP32 one keeps four
+// consecutive rows per lane group (row = 4*(l/16) + r).
+template <>
+struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int g, int r) { return 4 * g + r; }
+};
+
+template <typename T>
+struct GemmArgs {
+  int M, N, K;
+  const T* A;
+  long lda;
+  const T* B;
+  long ldb;
+  T* C;
+  long ldc;
+  T alpha, beta;
+  int mode, tri;
+  int tiles_m, tiles_n;
+  long sA, sB, sC;     // element strides between batch entries (blockIdx.y)
+  long sA2, sB2, sC2;  // second batch dimension (blockIdx.z)
+};
+
+constexpr int GROUP_M = 8;
+constexpr int SG = 8;
+constexpr int NXCD = 8;
+
+// Hardware hands workgroup ids to the XCDs round robin (id % 8). Renumber so
+// that XCD x sees a contiguous range of logical tile indices.
+__device__ __forceinline__ int xcd_linear(int bid, int nblk) {
+  int x = bid % NXCD;
+  int q = nblk / NXCD, r = nblk % NXCD;
+  return x * q + (x < r ? x : r) + bid / NXCD;
+}
+
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool TA, bool TB>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int LDS_A = BM + 4, LDS_B = BN + 4;
+  constexpr int A_PER = BM * BK / NT, B_PER = BN * BK / NT;
+  static_assert(A_PER * NT == BM * BK && B_PER * NT == BN * BK, "tile");
+  typedef typename Mfma<T>::acc_t acc_t;
+
+  __shared__ T As[2][BK * LDS_A];
+  __shared__ T Bs[2][BK * LDS_B];
+
+  // ---- tile selection ----
+  const int lin = xcd_linear(blockIdx.x, gridDim.x);
+  int tm, tn;
+  if (g.mode == GM_FULL) {
+    const int per_group = GROUP_M * g.tiles_n;
+    const int grp = lin / per_group;
+    const int first = grp * GROUP_M;
+    const int gsz = min(GROUP_M, g.tiles_m - first);
+    const int in = lin - grp * per_group;
+    tm = first + in % gsz;
+    tn = in / gsz;
+  } else {
+    // Triangle-only update: tiles are visited in SGxSG super-tiles (so the
+    // WGs that run together share row panels of op(A) and op(B) in L2);
+    // super-tiles are enumerated row by row over the lower triangle. Slots
+    // that fall outside the triangle exit at once.
+    const int sidx = lin / (SG * SG), w = lin % (SG * SG);
+    int r = (int)((sqrtf(8.0f * (float)sidx + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= sidx) ++r;
+    while (r * (r + 1) / 2 > sidx) --r;
+    const int c = sidx - r * (r + 1) / 2;
+    if (g.mode == GM_LOWER) {
+      tm = r * SG + w % SG;
+      tn = c * SG + w / SG;
+      if (tm >= g.tiles_m || tn > tm) return;
+    } else {
+      tm = c * SG + w % SG;
+      tn = r * SG + w / SG;
+      if (tn >= g.tiles_m || tm > tn) return;
+    }
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  int kbeg = 0, kend = g.K;
+  switch (g.tri) {
+    case TRI_A_LOWER: kend = min(g.K, m0 + BM); break;
+    case TRI_A_UPPER: kbeg = m0; break;
+    case TRI_B_LOWER: kbeg = n0; break;
+    case TRI_B_UPPER: kend = min(g.K, n0 + BN); break;
+    default: break;
+  }
+  kbeg = (kbeg / BK) * BK;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
+
+  T ra[A_PER], rb[B_PER];
+  const T* __restrict__ A = g.A + blockIdx.y * g.sA + blockIdx.z * g.sA2;
+  const T* __restrict__ B = g.B + blockIdx.y * g.sB + blockIdx.z * g.sB2;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb;
+
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < A_PER; ++e) {
+      const int idx = tid + e * NT;
+      int i, kk;
+      if (!TA) {
+        i = idx % BM;
+        kk = idx / BM;
+      } else {
+        kk = idx % BK;
+        i = idx / BK;
+      }
+      const int gi = m0 + i, gk = k0 + kk;
+      T v = T(0);
+      if (gi < M && gk < K) v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < B_PER; ++e) {
+      const int idx = tid + e * NT;
+      int j, kk;
+      if (TB) {
+        j = idx % BN;
+        kk = idx / BN;
+      } else {
+        kk = idx % BK;
+        j = idx / BK;
+      }
+      const int gj = n0 + j, gk = k0 + kk;
+      T v = T(0);
+      if (gj < N && gk < K) v = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
+      rb[e] = v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < A_PER; ++e) {
+      const int idx = tid + e * NT;
+      int i, kk;
+      if (!TA) {
+        i = idx % BM;
+        kk = idx / BM;
+      } else {
+        kk = idx % BK;
+        i = idx / BK;
+      }
+      As[buf][kk * LDS_A + i] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < B_PER; ++e) {
+      const int idx = tid + e * NT;
+      int j, kk;
+      if (TB) {
+        j = idx % BN;
+        kk = idx / BN;
+      } else {
+        kk = idx % BK;
+        j = idx / BK;
+      }
+      Bs[buf][kk * LDS_B + j] = rb[e];
+    }
+  };
+
+  if (kbeg < kend) {
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
+      if (more) load(k0 + BK);
+      const T* as = &As[buf][0];
+      const T* bs = &Bs[buf][0];
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = as[(kk + lk) * LDS_A + wm * WM + i * 16 + lr];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + lk) * LDS_B + wn * WN + j * 16 + lr];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
+      }
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+
+  // ---- epilogue: lane holds C[m0+..+lr][n0+..+drow(lk, r)] ----
+  const T alpha = g.alpha, beta = g.beta;
+  T* __restrict__ C = g.C + blockIdx.y * g.sC + blockIdx.z * g.sC2;
+  const long ldc = g.ldc;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int gm = m0 + wm * WM + i * 16 + lr;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gn = n0 + wn * WN + j * 16 + Mfma<T>::drow(lk, r);
+        bool ok = gm < M && gn < N;
+        if (g.mode == GM_LOWER) ok = ok && gm >= gn;
+        if (g.mode == GM_UPPER) ok = ok && gm <= gn;
+        if (ok) {
+          T* p = C + gm + (long)gn * ldc;
+          T v = alpha * acc[i][j][r];
+          if (beta != T(0)) v += beta * *p;
+          *p = v;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_scale_tile(int M, int N, T beta, T* C, long ldc) {
+  const long total = (long)M * N;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % M), j = (int)(t / M);
+    T* p = C + i + (long)j * ldc;
+    *p = beta == T(0) ? T(0) : beta * *p;
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_copy_tile(int M, int N, const S* A, long lda, D* B, long ldb) {
+  const long total = (long)M * N;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % M), j = (int)(t / M);
+    B[i + (long)j * ldb] = (D)A[i + (long)j * lda];
+  }
+}
+
+inline int grid_for(long total) {
+  long b = (total + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int batch2) {
+  g.tiles_m = ceil_div(g.M, BM);
+  g.tiles_n = ceil_div(g.N, BN);
+  long nblk;
+  if (g.mode == GM_FULL) {
+    nblk = (long)g.tiles_m * g.tiles_n;
+  } else {
+    const long ts = ceil_div(g.tiles_m, SG);
+    nblk = ts * (ts + 1) / 2 * SG * SG;
+  }
+  if (nblk <= 0) return ST_OK;
+  if (nblk > 0x7fffffff) {
+    set_error("gemm: grid too large");
+    return ST_BADARG;
+  }
+  dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)batch2), block(64 * WGM * WGN);
+  if (!ta && !tb)
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, g);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, g);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, g);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+template <typename T>
+int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, const T* A,
+              long lda, const T* B, long ldb, T beta, T* C, long ldc, int mode, int tri,
+              int batch = 1, long sA = 0, long sB = 0, long sC = 0, int batch2 = 1,
+              long sA2 = 0, long sB2 = 0, long sC2 = 0) {
+  if (M < 0 || N < 0 || K < 0) {
+    set_error("gemm: negative dimension");
+    return ST_BADARG;
+  }
+  if (M == 0 || N == 0 || batch <= 0 || batch2 <= 0) return ST_OK;
+  if (batch > 65535 || batch2 > 65535) {
+    set_error("gemm: batch too large");
+    return ST_BADARG;
+  }
+  if (mode != GM_FULL && M != N) {
+    set_error("gemm: triangular update needs a square C");
+    return ST_BADARG;
+  }
+  if (K == 0 || alpha == T(0)) {
+    if (beta == T(1)) return ST_OK;
+    if (batch != 1 || batch2 != 1) {
+      set_error("gemm: batched scale-only call is not supported");
+      return ST_BADARG;
+    }
+    if (mode != GM_FULL) {
+      // triangle-only scale: rare, fall back to per-column launches
+      for (int j = 0; j < N; ++j) {
+        int r0 = mode == GM_LOWER ? j : 0;
+        int rows = mode == GM_LOWER ? M - j : j + 1;
+        hipLaunchKernelGGL(k_scale_tile<T>, dim3(grid_for(rows)), dim3(256), 0, s, rows, 1,
+                           beta, C + r0 + (long)j * ldc, ldc);
+      }
+    } else {
+      hipLaunchKernelGGL(k_scale_tile<T>, dim3(grid_for((long)M * N)), dim3(256), 0, s, M, N,
+                         beta, C, ldc);
+    }
+    MOPS_CHECK_HIP(hipGetLastError());
+    return ST_OK;
+  }
+  GemmArgs<T> g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.mode = mode;
+  g.tri = tri;
+  g.sA = sA;
+  g.sB = sB;
+  g.sC = sC;
+  g.sA2 = sA2;
+  g.sB2 = sB2;
+  g.sC2 = sC2;
+  // Large outputs: 128x128 tiles (16 MFMA blocks per wave). Smaller ones use
+  // 64x64 tiles so that the grid still covers the 256 CUs.
+  const long t128 = (long)ceil_div(M, 128) * ceil_div(N, 128) * batch * batch2;
+  if (t128 >= 512) return launch_gemm<T, 128, 128, 16, 2, 4>(s, ta, tb, g, batch, batch2);
+  return launch_gemm<T, 64, 64, 32, 2, 2>(s, ta, tb, g, batch, batch2);
+}
+
+}  // namespace
+
+int gemm_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, const double* A,
+           long lda, const double* B, long ldb, double beta, double* C, long ldc, int mode,
+           int tri) {
+  return gemm_impl<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, tri);
+}
+
+int gemm_s(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alpha, const float* A,
+           long lda, const float* B, long ldb, float beta, float* C, long ldc, int mode,
+           int tri) {
+  return gemm_impl<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, mode, tri);
+}
+
+int gemm_batched_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                   const double* A, long lda, long sA, const double* B, long ldb, long sB,
+                   double beta, double* C, long ldc, long sC, int batch, int tri) {
+  return gemm_impl<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, GM_FULL, tri,
+                           batch, sA, sB, sC);
+}
+
+int gemm_batched2_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
+                    const double* A, long lda, long sA, long sA2, const double* B, long ldb,
+                    long sB, long sB2, double beta, double* C, long ldc, long sC, long sC2,
+                    int batch, int batch2, int tri) {
+  return gemm_impl<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, GM_FULL, tri,
+                           batch, sA, sB, sC, batch2, sA2, sB2, sC2);
+}
+
+int gemm_batched_s(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alpha,
+                   const float* A, long lda, long sA, const float* B, long ldb, long sB,
+                   float beta, float* C, long ldc, long sC, int batch, int tri) {
+  return gemm_impl<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, GM_FULL, tri,
+                          batch, sA, sB, sC);
+}
+
+int scale_tile_d(hipStream_t s, int M, int N, double beta, double* C, long ldc) {
+  if (M <= 0 || N <= 0 || beta == 1.0) return ST_OK;
+  hipLaunchKernelGGL(k_scale_tile<double>, dim3(grid_for((long)M * N)), dim3(256), 0, s, M, N,
+                     beta, C, ldc);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int copy_tile_d(hipStream_t s, int M, int N, const double* A, long lda, double* B, long ldb) {
+  if (M <= 0 || N <= 0) return ST_OK;
+  hipLaunchKernelGGL((k_copy_tile<double, double>), dim3(grid_for((long)M * N)), dim3(256), 0, s,
+                     M, N, A, lda, B, ldb);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int copy_tile_s(hipStream_t s, int M, int N, const float* A, long lda, float* B, long ldb) {
+  if (M <= 0 || N <= 0) return ST_OK;
+  hipLaunchKernelGGL((k_copy_tile<float, float>), dim3(grid_for((long)M * N)), dim3(256), 0, s,
+                     M, N, A, lda, B, ldb);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int convert_d2s(hipStream_t s, int M, int N, const double* A, long lda, float* B, long ldb) {
+  if (M <= 0 || N <= 0) return ST_OK;
+  hipLaunchKernelGGL((k_copy_tile<double, float>), dim3(grid_for((long)M * N)), dim3(256), 0, s,
+                     M, N, A, lda, B, ldb);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int convert_s2d(hipStream_t s, int M, int N, const float* A, long lda, double* B, long ldb) {
+  if (M <= 0 || N <= 0) return ST_OK;
+  hipLaunchKernelGGL((k_copy_tile<float, double>), dim3(grid_for((long)M * N)), dim3(256), 0, s,
+                     M, N, A, lda, B, ldb);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+}  // namespace mops

@@ -1,0 +1,97 @@
+// Shared definitions for the gfx950 dense FP64 path: flag bits, error state,
+// per-thread device workspace and small device helpers.
+//
+// Flag bits keep the values of the reference's cgo boundary
+// (calgo/cmops.h:13-24, calgo/cmops.go:19-31) so a caller can pass the
+// reference's Flags value through unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mops {
+
+enum : int {
+  F_NOTRANS = 0,
+  F_TRANSA = 0x1,
+  F_TRANSB = 0x2,
+  F_LOWER = 0x4,
+  F_UPPER = 0x8,
+  F_LEFT = 0x10,
+  F_RIGHT = 0x20,
+  F_UNIT = 0x40,
+  F_TRANS = 0x80,
+  F_NOSCALE = 0x800,
+};
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Status codes returned through the C ABI.
+enum : int {
+  ST_OK = 0,
+  ST_BADARG = -1,     // negative: -(index of the bad argument) when known
+  ST_HIP = -1000,     // HIP runtime failure, message in mops_last_error()
+  ST_NOMEM = -1001,   // device allocation failed
+  ST_NOTSPD = -2000,  // Cholesky met a NaN square root (info holds the index)
+};
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+const char* last_error();
+
+#define MOPS_CHECK_HIP(call)                                                  \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::mops::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,   \
+                        hipGetErrorString(e_));                               \
+      return ::mops::ST_HIP;                                                  \
+    }                                                                         \
+  } while (0)
+
+#define MOPS_TRY(expr)        \
+  do {                        \
+    int st_ = (expr);         \
+    if (st_ != 0) return st_; \
+  } while (0)
+
+// Device workspace owned by the calling host thread, one per device. It only
+// grows; requests are served from a bump region that is reset by the
+// top-level entry point, so nested helpers can carve sub-buffers.
+struct Workspace {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  int device = -1;
+};
+
+// Returns the calling thread's workspace for the current device, grown to at
+// least `bytes` of free space beyond what is already carved. Growing discards
+// previously carved buffers, so callers reserve once up front (reserve()) and
+// then carve().
+int ws_reserve(size_t bytes);
+void* ws_carve(size_t bytes);  // 256-byte aligned; nullptr when out of space
+void ws_reset();
+// Frees the calling thread's workspace and streams on every device.
+void thread_release();
+
+// Per-thread stream used by the host-pointer entry points.
+hipStream_t host_stream();
+
+// Optional timing of selected launches with HIP events recorded on the
+// launching stream (per host thread). Classes name what is timed.
+enum : int { TR_SYRK_UPDATE = 0, TR_GEMM_UPDATE = 1, TR_PANEL = 2, TR_NCLASS = 3 };
+bool trace_on();
+// Records an event pair around a launch sequence: call trace_mark_begin
+// before and trace_mark_end (with the algorithmic flops) after.
+void trace_mark_begin(hipStream_t s, int cls);
+void trace_mark_end(hipStream_t s, int cls, double flops);
+void trace_enable(int on);
+int trace_collect(int cls, int* count, double* ms, double* flops);
+
+static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+static inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace mops

@@ -1,0 +1,47 @@
+// Device-side drivers of the factorizations and triangular solves.
+// All pointers are device pointers; work is queued on `s`.
+#pragma once
+
+#include "mops_kernels.hpp"
+
+namespace mops {
+
+// Bytes of workspace trtri_d needs beyond X for an n x n triangle.
+size_t trtri_ws_bytes(int n);
+// X (n x n, dense, zeros in the other triangle) = inverse of the lower
+// (lower=true) or upper triangle of A; unit: the diagonal is taken as 1 and
+// never read. Inverts all 64x64 diagonal blocks in one launch, then combines
+// them level by level with batched GEMMs. Uses workspace (carve).
+int trtri_d(hipStream_t s, bool lower, bool unit, int n, const double* A, long lda, double* X,
+            long ldx);
+
+// Inverse of every bs x bs diagonal block of the n x n triangle of A
+// (the last block may be smaller), stored block b at X + b*bs*bs with
+// leading dimension bs.
+int trtri_blocks_d(hipStream_t s, bool lower, bool unit, int n, int bs, const double* A,
+                   long lda, double* X);
+
+// Workspace needed by trsm_d.
+size_t trsm_ws_bytes(int flags, int N, int nrhs);
+// B = alpha * op(A)^-1 * B (LEFT, B is N x nrhs) or alpha * B * op(A)^-1
+// (RIGHT, B is nrhs x N). Flag semantics of dmmat_solve_blk
+// (calgo/dmmat_solve.c:864-909): RIGHT selects the side, otherwise LEFT;
+// UPPER selects the upper triangle, otherwise lower; TRANSA transposes A;
+// UNIT never reads the diagonal.
+int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
+           double* B, long ldb);
+
+// Cholesky of the N x N matrix A in place (LOWER or UPPER flag), outer block
+// nb. `info` (device int, may be null) receives the 0-based global index of
+// the first NaN square root, or stays -1. `badval` (device double, may be
+// null) receives the value under that square root.
+size_t potrf_ws_bytes(int N, int nb);
+int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* info,
+            double* badval);
+
+// B = A^-1 B from the Cholesky factor (SolveCHOL, chol.go:147-158).
+size_t potrs_ws_bytes(int flags, int N, int nrhs);
+int potrs_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda, double* B,
+            long ldb);
+
+}  // namespace mops

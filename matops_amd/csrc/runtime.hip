@@ -1,0 +1,179 @@
+// Per-thread runtime state behind the C ABI: last-error text, a growable
+// device workspace and a stream per host thread and device.
+//
+// The reference calls its kernels concurrently from several goroutines on
+// disjoint blocks of C (mmult.go:241-255), so nothing here is shared between
+// host threads: each thread owns its stream and workspace.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "mops_common.hpp"
+
+namespace mops {
+
+namespace {
+thread_local char g_err[512] = {0};
+
+struct ThreadState {
+  Workspace ws[16];
+  hipStream_t stream[16] = {nullptr};
+};
+thread_local ThreadState g_ts;
+
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) return 0;
+  return d < 0 || d >= 16 ? 0 : d;
+}
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+void clear_error() { g_err[0] = 0; }
+
+const char* last_error() { return g_err; }
+
+int ws_reserve(size_t bytes) {
+  const int d = current_device();
+  Workspace& w = g_ts.ws[d];
+  bytes = align_up(bytes, 256) + 256;
+  if (w.base && w.cap >= bytes) {
+    w.used = 0;
+    return ST_OK;
+  }
+  if (w.base) {
+    // the old buffer may still be read by queued work on this thread's streams
+    (void)hipDeviceSynchronize();
+    (void)hipFree(w.base);
+    w.base = nullptr;
+    w.cap = 0;
+  }
+  size_t cap = bytes < (size_t(64) << 20) ? (size_t(64) << 20) : bytes;
+  void* p = nullptr;
+  if (hipMalloc(&p, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device workspace of %zu bytes could not be allocated", cap);
+    return ST_NOMEM;
+  }
+  w.base = static_cast<char*>(p);
+  w.cap = cap;
+  w.used = 0;
+  w.device = d;
+  return ST_OK;
+}
+
+void* ws_carve(size_t bytes) {
+  Workspace& w = g_ts.ws[current_device()];
+  size_t off = align_up(w.used, 256);
+  bytes = align_up(bytes, 256);
+  if (!w.base || off + bytes > w.cap) return nullptr;
+  w.used = off + bytes;
+  return w.base + off;
+}
+
+void ws_reset() { g_ts.ws[current_device()].used = 0; }
+
+void thread_release() {
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (int d = 0; d < 16; ++d) {
+    if (!g_ts.ws[d].base && !g_ts.stream[d]) continue;
+    if (hipSetDevice(d) != hipSuccess) continue;
+    if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
+    if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
+    if (g_ts.stream[d]) (void)hipStreamDestroy(g_ts.stream[d]);
+    g_ts.ws[d] = Workspace();
+    g_ts.stream[d] = nullptr;
+  }
+  (void)hipSetDevice(cur);
+}
+
+namespace {
+struct TraceRec {
+  hipEvent_t a, b;
+  int cls;
+  double flops;
+};
+struct TraceState {
+  bool on = false;
+  int open_cls = -1;
+  hipEvent_t open_ev = nullptr;
+  TraceRec* recs = nullptr;
+  int n = 0, cap = 0;
+};
+thread_local TraceState g_tr;
+}  // namespace
+
+bool trace_on() { return g_tr.on; }
+
+void trace_mark_begin(hipStream_t s, int cls) {
+  if (!g_tr.on) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  g_tr.open_ev = e;
+  g_tr.open_cls = cls;
+}
+
+void trace_mark_end(hipStream_t s, int cls, double flops) {
+  if (!g_tr.on || g_tr.open_cls != cls || !g_tr.open_ev) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  if (g_tr.n == g_tr.cap) {
+    int nc = g_tr.cap ? 2 * g_tr.cap : 256;
+    TraceRec* nr = new TraceRec[nc];
+    for (int i = 0; i < g_tr.n; ++i) nr[i] = g_tr.recs[i];
+    delete[] g_tr.recs;
+    g_tr.recs = nr;
+    g_tr.cap = nc;
+  }
+  g_tr.recs[g_tr.n++] = TraceRec{g_tr.open_ev, e, cls, flops};
+  g_tr.open_ev = nullptr;
+  g_tr.open_cls = -1;
+}
+
+void trace_enable(int on) {
+  for (int i = 0; i < g_tr.n; ++i) {
+    (void)hipEventDestroy(g_tr.recs[i].a);
+    (void)hipEventDestroy(g_tr.recs[i].b);
+  }
+  g_tr.n = 0;
+  g_tr.on = on != 0;
+}
+
+int trace_collect(int cls, int* count, double* ms, double* flops) {
+  int c = 0;
+  double t = 0.0, f = 0.0;
+  for (int i = 0; i < g_tr.n; ++i) {
+    if (g_tr.recs[i].cls != cls) continue;
+    float e = 0.f;
+    if (hipEventSynchronize(g_tr.recs[i].b) != hipSuccess) return ST_HIP;
+    if (hipEventElapsedTime(&e, g_tr.recs[i].a, g_tr.recs[i].b) != hipSuccess) return ST_HIP;
+    ++c;
+    t += e;
+    f += g_tr.recs[i].flops;
+  }
+  *count = c;
+  *ms = t;
+  *flops = f;
+  return ST_OK;
+}
+
+hipStream_t host_stream() {
+  const int d = current_device();
+  if (!g_ts.stream[d]) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    g_ts.stream[d] = s;
+  }
+  return g_ts.stream[d];
+}
+
+}  // namespace mops

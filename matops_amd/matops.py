@@ -1,0 +1,185 @@
+"""The matops API of the dense FP64 path, on top of the C ABI.
+
+Mirrors the Go functions the path consists of -- names, argument meaning,
+validation and error behaviour -- with column-major numpy views standing in
+for ``matrix.FloatMatrix``: any 2-D float64 array whose rows are contiguous
+(``strides[0] == 8``) is a view; its leading dimension is
+``strides[1] // 8``, exactly the (pointer at (0,0), step) pair calgo gets
+(calgo/cmops.go:68-91). Sub-views of a Fortran-ordered array
+(``A[r0:r1, c0:c1]``) therefore share memory with their parent the way
+FloatMatrix.SubMatrix does.
+
+Everything here computes on the GPU through libmops.so; there is no CPU
+fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import MData, check
+
+TRANSA = 0x1
+TRANSB = 0x2
+LOWER = 0x4
+UPPER = 0x8
+LEFT = 0x10
+RIGHT = 0x20
+UNIT = 0x40
+TRANS = 0x80
+NOTRANS = 0
+NONE = 0
+
+# CPU blocking hints the reference passes down (mmult.go:33-35). The GPU
+# ignores them; they are kept so BlockingParams keeps its meaning.
+_vpLen, _nB, _mB = 196, 68, 68
+_nWorker = 1
+_panicOnError = False
+
+
+class MatopsError(Exception):
+    """The error value a Go call would return."""
+
+
+def SetPanicOnError(newval):
+    """mmult.go:55-57. With panic-on-error, errors raise RuntimeError
+    (Go panic) instead of being returned."""
+    global _panicOnError
+    _panicOnError = bool(newval)
+
+
+def BlockingParams(mb, nb, kb):
+    """mmult.go:63-67 (hints only on the GPU)."""
+    global _vpLen, _nB, _mB
+    _vpLen, _nB, _mB = kb, nb, mb
+
+
+def NumWorkers(newWorkers):
+    """mmult.go:69-73. Returns the previous value. The GPU path does not
+    split work over host threads, so the value is only remembered."""
+    global _nWorker
+    old = _nWorker
+    _nWorker = newWorkers
+    return old
+
+
+def _onError(msg):
+    # mmult.go:46-51
+    if _panicOnError:
+        raise RuntimeError(msg)
+    return MatopsError(msg)
+
+
+def _view(X):
+    """(MData, rows, cols) of a column-major float64 view."""
+    if not isinstance(X, np.ndarray) or X.dtype != np.float64 or X.ndim != 2:
+        raise TypeError("matrix views are 2-D float64 numpy arrays")
+    rows, cols = X.shape
+    if rows > 1 and X.strides[0] != 8:
+        raise ValueError("view rows must be contiguous (column-major storage)")
+    ld = X.strides[1] // 8 if cols > 1 else max(rows, 1)
+    if cols > 1 and (X.strides[1] % 8 or ld < rows):
+        raise ValueError("bad column stride for a column-major view")
+    return MData(ctypes.c_void_p(X.ctypes.data), max(ld, 1)), rows, cols
+
+
+def Mult(C, A, B, alpha, beta, flags):
+    """C = beta*C + alpha*op(A)*op(B)  (mmult.go:198-256)."""
+    ar, ac = A.shape
+    br, bc = B.shape
+    cr, cc = C.shape
+    tt = flags & (TRANSA | TRANSB)
+    if tt == TRANSA | TRANSB:
+        empty = ac == 0 or br == 0
+        ok = cr == ac and cc == br and ar == bc
+    elif tt == TRANSA:
+        empty = ac == 0 or bc == 0
+        ok = cr == ac and cc == bc and ar == br
+    elif tt == TRANSB:
+        empty = ar == 0 or br == 0
+        ok = cr == ar and cc == br and ac == bc
+    else:
+        empty = ar == 0 or bc == 0
+        ok = cr == ar and cc == bc and ac == br
+    if empty:
+        return None
+    if not ok:
+        return MatopsError("Mult: size mismatch")
+    P = A.shape[0] if flags & TRANSA else A.shape[1]
+    Cm, _, _ = _view(C)
+    Am, _, _ = _view(A)
+    Bm, _, _ = _view(B)
+    check("mops_dmult_mm_blocked3", _lib.lib().mops_dmult_mm_blocked3(
+        ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags,
+        P, 0, cc, 0, cr, _vpLen, _nB, _mB))
+    return None
+
+
+def SolveTrm(B, A, alpha, flags):
+    """B = alpha*op(A)^-1*B or alpha*B*op(A)^-1  (mmult.go:388-422)."""
+    br, bc = B.shape
+    ar, ac = A.shape
+    empty, ok = False, True
+    side = flags & (LEFT | RIGHT)
+    if side == LEFT:
+        empty = br == 0
+        ok = br == ac and ac == ar
+    elif side == RIGHT:
+        empty = bc == 0
+        ok = bc == ar and ac == ar
+    if empty:
+        return None
+    if not ok:
+        return _onError("A, B size mismatch")
+    E = br if flags & RIGHT else bc
+    # calgo.DSolveBlk returns early when N == 0 or E-S <= 0 (cmops.go:203)
+    if ac == 0 or E <= 0:
+        return None
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dmmat_solve_blk", _lib.lib().mops_dmmat_solve_blk(
+        ctypes.byref(Bm), ctypes.byref(Am), alpha, flags, ac, 0, E, _nB))
+    return None
+
+
+def RankUpdateSym(C, A, alpha, beta, flags):
+    """C = beta*C + alpha*A*A^T (alpha*A^T*A with TRANSA) on one triangle
+    (mmult.go:426-445)."""
+    if C.shape[0] != C.shape[1]:
+        return _onError("C not a square matrix")
+    P = A.shape[0] if flags & TRANSA else A.shape[1]
+    E = C.shape[0]
+    if P == 0 or E == 0:
+        return None
+    Cm, _, _ = _view(C)
+    Am, _, _ = _view(A)
+    check("mops_dmmat_rank_blk", _lib.lib().mops_dmmat_rank_blk(
+        ctypes.byref(Cm), ctypes.byref(Am), alpha, beta, flags, P, 0, E, _vpLen, _nB))
+    return None
+
+
+def DecomposeCHOL(A, flags, nb):
+    """Cholesky in place (chol.go:116-127). Returns (A, err). A NaN square
+    root panics with the reference's message (chol.go:33-36)."""
+    if A.shape[0] != A.shape[1]:
+        return A, MatopsError("A not a square matrix")
+    n = A.shape[0]
+    if n == 0:
+        return A, None
+    Am, _, _ = _view(A)
+    rc = _lib.lib().mops_dpotrf(ctypes.byref(Am), flags, n, nb)
+    if rc > 0:
+        raise RuntimeError(_lib.last_error())  # Go panic
+    check("mops_dpotrf", rc)
+    return A, None
+
+
+def SolveCHOL(B, A, flags):
+    """B = A^-1 B from the Cholesky factor (chol.go:147-158)."""
+    n = A.shape[0]
+    if n == 0 or B.shape[1] == 0:
+        return
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dpotrs", _lib.lib().mops_dpotrs(
+        ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
