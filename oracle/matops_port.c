@@ -1,0 +1,507 @@
+/*
+ * The Go half of matops' dense FP64 path restated in C -- TEST
+ * INFRASTRUCTURE (the oracle). Go is not available to build the reference,
+ * so its glue (views, partitioning, pivots, the blocked/unblocked
+ * algorithms and the cgo calls) is restated here statement by statement;
+ * each function cites what it follows. The kernels are reached through the
+ * okern_t table (the port in calgo_port.c, or the reference's calgo C
+ * loaded from oracle/_ref).
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <math.h>
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "oracle.h"
+
+static okern_t K;
+static int K_init = 0;
+static int K_which = 0;
+
+static const okern_t *kern(void) {
+  if (!K_init) {
+    K = o_port_kernels;
+    K_init = 1;
+  }
+  return &K;
+}
+
+int o_use_kernels(int which, const char *path) {
+  if (which == 0) {
+    K = o_port_kernels;
+    K_init = 1;
+    K_which = 0;
+    return 0;
+  }
+  void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return -1;
+  okern_t k;
+#define SYM(name)                                    \
+  *(void **)(&k.name) = dlsym(h, #name);             \
+  if (!k.name) return -1;
+  SYM(dmult_mm_blocked3)
+  SYM(dmmat_solve_blk)
+  SYM(dmmat_rank_blk)
+  SYM(dmvec_symv_rank)
+  SYM(dmvec_solve_blocked)
+  SYM(dmult_gemv_blocked)
+  SYM(dvec_dot)
+  SYM(dvec_swap)
+  SYM(dvec_invscal)
+#undef SYM
+  K = k;
+  K_init = 1;
+  K_which = 1;
+  return 0;
+}
+
+int o_kernels_in_use(void) { return K_which; }
+
+/* Mixes the two kernel sets: bit i of `mask` takes the i-th okern_t entry
+ * from the reference (0 dmult_mm_blocked3, 1 dmmat_solve_blk, 2
+ * dmmat_rank_blk, 3 dmvec_symv_rank, 4 dmvec_solve_blocked, 5
+ * dmult_gemv_blocked, 6 dvec_dot, 7 dvec_swap, 8 dvec_invscal). Used to
+ * localise differences between the restatement and the reference. */
+int o_use_kernel_mask(int mask, const char *path) {
+  if (o_use_kernels(1, path) != 0) return -1;
+  okern_t ref = K;
+  okern_t mix = o_port_kernels;
+  void **dst = (void **)&mix;
+  void **src = (void **)&ref;
+  for (int i = 0; i < (int)(sizeof(okern_t) / sizeof(void *)); i++)
+    if (mask & (1 << i)) dst[i] = src[i];
+  K = mix;
+  K_which = mask ? 2 : 0;
+  return 0;
+}
+
+double o_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ---- tuning globals, mmult.go:33-44, 63-73 ---- */
+static int vpLen = 196, nB = 68, mB = 68;
+static int nWorker = 1;
+static const int64_t limitOne = 200 * 200 * 200;
+
+void o_BlockingParams(int mb, int nb, int kb) {
+  vpLen = kb;
+  nB = nb;
+  mB = mb;
+}
+
+int o_NumWorkers(int w) {
+  int old = nWorker;
+  nWorker = w;
+  return old;
+}
+
+/* ---- views (hrautila/matrix FloatMatrix semantics) ---- */
+static oview sub(oview A, int r, int c, int rows, int cols) {
+  oview v = {A.p + r + (ptrdiff_t)c * A.ld, rows, cols, A.ld};
+  return v;
+}
+static int numel(oview A) { return A.rows * A.cols; }
+static o_mdata_t md(oview A) {
+  o_mdata_t m = {A.p, A.ld};
+  return m;
+}
+/* row vectors step by the leading index, column vectors by 1 */
+static o_mvec_t mv_rows(oview X) { /* MVMult & co: X.Rows() == 1 -> inc = ld (mvec.go:45-48) */
+  o_mvec_t v = {X.p, X.rows == 1 ? X.ld : 1};
+  return v;
+}
+static o_mvec_t mv_cols(oview X) { /* Dot, InvScale, Swap: X.Cols() != 1 -> inc = ld (mvec.go:286) */
+  o_mvec_t v = {X.p, X.cols != 1 ? X.ld : 1};
+  return v;
+}
+
+/* ---- Mult, mmult.go:198-256 with divideWork/scheduleWork (:137-188) ---- */
+static int blockIndex4(int i, int r, int sz) { /* mmult.go:105-110 */
+  if (i == r) return sz;
+  return i * sz / r - ((i * sz / r) & 0x3);
+}
+
+static void divideWork(int rows, int cols, int workers, int *colWorkers, int *rowWorkers) {
+  /* mmult.go:137-169 */
+  int nwsqrt = (int)sqrt((double)workers);
+  int issquare = nwsqrt * nwsqrt == workers;
+  if (workers == 2 || (workers & 0x1) != 0) {
+    if (cols > rows) {
+      *colWorkers = workers;
+      *rowWorkers = 1;
+    } else {
+      *rowWorkers = workers;
+      *colWorkers = 1;
+    }
+  } else if (issquare) {
+    *colWorkers = nwsqrt;
+    *rowWorkers = nwsqrt;
+  } else {
+    if (cols > rows) {
+      *rowWorkers = 2;
+      *colWorkers = workers / 2;
+    } else {
+      *colWorkers = 2;
+      *rowWorkers = workers / 2;
+    }
+  }
+}
+
+typedef struct {
+  o_mdata_t C, A, B;
+  double alpha, beta;
+  int flags, P, S, L, R, E;
+} mult_task;
+
+static void *mult_worker(void *arg) {
+  mult_task *t = (mult_task *)arg;
+  kern()->dmult_mm_blocked3(&t->C, &t->A, &t->B, t->alpha, t->beta, t->flags, t->P, t->S, t->L,
+                            t->R, t->E, vpLen, nB, mB);
+  return NULL;
+}
+
+int o_Mult(oview C, oview A, oview B, double alpha, double beta, int flags) {
+  int ar = A.rows, ac = A.cols, br = B.rows, bc = B.cols, cr = C.rows, cc = C.cols;
+  int empty, ok;
+  switch (flags & (O_TRANSA | O_TRANSB)) {
+    case O_TRANSA | O_TRANSB:
+      empty = ac == 0 || br == 0;
+      ok = cr == ac && cc == br && ar == bc;
+      break;
+    case O_TRANSA:
+      empty = ac == 0 || bc == 0;
+      ok = cr == ac && cc == bc && ar == br;
+      break;
+    case O_TRANSB:
+      empty = ar == 0 || br == 0;
+      ok = cr == ar && cc == br && ac == bc;
+      break;
+    default:
+      empty = ar == 0 || bc == 0;
+      ok = cr == ar && cc == bc && ac == br;
+  }
+  if (empty) return 0;
+  if (!ok) return 1; /* "Mult: size mismatch" */
+  int64_t psize = (int64_t)numel(C) * (int64_t)A.cols;
+  int P = flags & O_TRANSA ? A.rows : A.cols;
+  o_mdata_t Cm = md(C), Am = md(A), Bm = md(B);
+  if (nWorker <= 1 || psize <= limitOne) {
+    kern()->dmult_mm_blocked3(&Cm, &Am, &Bm, alpha, beta, flags, P, 0, C.cols, 0, C.rows, vpLen,
+                              nB, mB);
+    return 0;
+  }
+  int colworks, rowworks;
+  divideWork(C.rows, C.cols, nWorker, &colworks, &rowworks);
+  int ntask = colworks * rowworks;
+  pthread_t *th = malloc(sizeof(pthread_t) * ntask);
+  mult_task *tk = malloc(sizeof(mult_task) * ntask);
+  int n = 0;
+  for (int k = 0; k < colworks; k++) {
+    int colstart = blockIndex4(k, colworks, C.cols), colend = blockIndex4(k + 1, colworks, C.cols);
+    for (int l = 0; l < rowworks; l++) {
+      int rowstart = blockIndex4(l, rowworks, C.rows), rowend = blockIndex4(l + 1, rowworks, C.rows);
+      mult_task t = {Cm, Am, Bm, alpha, beta, flags, P, colstart, colend, rowstart, rowend};
+      tk[n] = t;
+      pthread_create(&th[n], NULL, mult_worker, &tk[n]);
+      n++;
+    }
+  }
+  for (int i = 0; i < n; i++) pthread_join(th[i], NULL);
+  free(th);
+  free(tk);
+  return 0;
+}
+
+/* ---- SolveTrm, mmult.go:388-422 -> calgo.DSolveBlk, calgo/cmops.go:195-214 ---- */
+int o_SolveTrm(oview B, oview A, double alpha, int flags) {
+  int ok = 1, empty = 0;
+  int br = B.rows, bc = B.cols, ar = A.rows, ac = A.cols;
+  switch (flags & (O_LEFT | O_RIGHT)) {
+    case O_LEFT:
+      empty = br == 0;
+      ok = br == ac && ac == ar;
+      break;
+    case O_RIGHT:
+      empty = bc == 0;
+      ok = bc == ar && ac == ar;
+      break;
+  }
+  if (empty) return 0;
+  if (!ok) return 1; /* "A, B size mismatch" */
+  int E = flags & O_RIGHT ? br : bc;
+  if (ac == 0 || E <= 0) return 0; /* cmops.go:203 */
+  o_mdata_t Bm = md(B), Am = md(A);
+  kern()->dmmat_solve_blk(&Bm, &Am, alpha, flags, ac, 0, E, nB);
+  return 0;
+}
+
+/* ---- RankUpdateSym, mmult.go:426-445 -> DSymmRankBlk cmops.go:219-240 ---- */
+int o_RankUpdateSym(oview C, oview A, double alpha, double beta, int flags) {
+  if (C.rows != C.cols) return 1; /* "C not a square matrix" */
+  int P = flags & O_TRANSA ? A.rows : A.cols;
+  int E = C.rows;
+  if (P == 0 || E <= 0) return 0; /* cmops.go:227 */
+  o_mdata_t Cm = md(C), Am = md(A);
+  kern()->dmmat_rank_blk(&Cm, &Am, alpha, beta, flags, P, 0, E, vpLen, nB);
+  return 0;
+}
+
+/* ---- vector helpers (mvec.go) ---- */
+static void MVMult(oview Y, oview A, oview X, double alpha, double beta, int flags) {
+  /* mvec.go:28-60 */
+  if (A.rows == 0 || A.cols == 0) return;
+  o_mvec_t Yv = mv_rows(Y), Xv = mv_rows(X);
+  o_mdata_t Am = md(A);
+  kern()->dmult_gemv_blocked(&Yv, &Am, &Xv, alpha, beta, flags, 0, numel(X), 0, numel(Y), vpLen,
+                             mB);
+}
+
+static void MVRankUpdateSym(oview A, oview X, double alpha, int flags) {
+  /* mvec.go:101-121 -> DSymmRankMV(..., S=0, L=A.Cols(), NB=0) */
+  if (A.rows == 0 || A.cols == 0) return;
+  o_mvec_t Xv = mv_rows(X);
+  o_mdata_t Am = md(A);
+  kern()->dmvec_symv_rank(&Am, &Xv, alpha, flags, 0, A.cols, 0);
+}
+
+static void MVSolveTrm(oview X, oview A, int flags) {
+  /* mvec.go:203-222 -> DSolveBlkMV(X, A, flags, incX, ldA, N=A.Cols(), nB) */
+  if (A.rows == 0 || A.cols == 0) return;
+  o_mvec_t Xv = mv_rows(X);
+  o_mdata_t Am = md(A);
+  kern()->dmvec_solve_blocked(&Xv, &Am, flags, A.cols, nB);
+}
+
+static double Dot(oview X, oview Y, double alpha) {
+  /* mvec.go:274-297 -> DDot: N <= 0 returns 0 (cmops.go:595-597) */
+  int N = numel(X);
+  if (N <= 0) return 0.0;
+  o_mvec_t Xv = mv_cols(X), Yv = mv_cols(Y);
+  return kern()->dvec_dot(&Xv, &Yv, alpha, N);
+}
+
+static void InvScale(oview X, double alpha) {
+  /* mvec.go:422-437 */
+  if (numel(X) == 0) return;
+  if (X.rows != 1 && X.cols != 1) return;
+  o_mvec_t Xv = mv_cols(X);
+  kern()->dvec_invscal(&Xv, alpha, numel(X));
+}
+
+static void Swap(oview X, oview Y) {
+  /* mvec.go:456-482 */
+  if (numel(X) == 0 || numel(Y) == 0) return;
+  o_mvec_t Xv = mv_cols(X), Yv = mv_cols(Y);
+  kern()->dvec_swap(&Xv, &Yv, numel(X));
+}
+
+/* ---- pivots, pivot.go ---- */
+static void swapRows(oview A, int src, int dst) { /* pivot.go:27-35 */
+  if (src == dst || A.rows == 0) return;
+  Swap(sub(A, src, 0, 1, A.cols), sub(A, dst, 0, 1, A.cols));
+}
+
+static void applyPivots(oview A, const int64_t *p, int np) { /* pivot.go:55-61 */
+  for (int k = 0; k < np; k++)
+    if (p[k] > 0) swapRows(A, (int)p[k], k);
+}
+
+static void pivotIndex(oview A, int64_t *p) { /* pivot.go:98-107 */
+  double max = fabs(A.p[0]);
+  for (int k = 1; k < A.rows; k++) {
+    double v = fabs(A.p[k]);
+    if (v > max) {
+      p[0] = k;
+      max = v;
+    }
+  }
+}
+
+/* ---- Cholesky, chol.go ---- */
+static int unblockedCHOL(oview A, int flags, int nr, int *pidx, double *pval) {
+  /* chol.go:17-56: per column a11 = sqrt(a11) (NaN -> panic), a21 /= a11,
+   * A22 -= a21 a21^T (LOWER), or the a12 row version (UPPER) */
+  for (int k = 0; k < A.rows; k++) {
+    oview a11 = sub(A, k, k, 1, 1);
+    double aval = sqrt(a11.p[0]);
+    if (isnan(aval)) {
+      *pidx = nr + k;
+      *pval = a11.p[0];
+      return 2;
+    }
+    a11.p[0] = aval;
+    if (flags & O_LOWER) {
+      oview a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+      InvScale(a21, aval);
+      MVRankUpdateSym(sub(A, k + 1, k + 1, A.rows - k - 1, A.cols - k - 1), a21, -1.0, flags);
+    } else {
+      oview a12 = sub(A, k, k + 1, 1, A.cols - k - 1);
+      InvScale(a12, aval);
+      MVRankUpdateSym(sub(A, k + 1, k + 1, A.rows - k - 1, A.cols - k - 1), a12, -1.0, flags);
+    }
+  }
+  return 0;
+}
+
+static int blockedCHOL(oview A, int flags, int nb, int *pidx, double *pval) {
+  /* chol.go:58-94 */
+  for (int k = 0; k < A.rows && k < A.cols;) {
+    int kb = k + nb > A.cols ? A.cols - k : nb; /* partition.go:252-254 */
+    oview A11 = sub(A, k, k, kb, kb);
+    int e = unblockedCHOL(A11, flags, k, pidx, pval);
+    if (e) return e;
+    int m = A.rows - k - kb;
+    if (flags & O_LOWER) {
+      oview A21 = sub(A, k + kb, k, m, kb);
+      o_SolveTrm(A21, A11, 1.0, O_RIGHT | O_LOWER | O_TRANSA);
+      o_RankUpdateSym(sub(A, k + kb, k + kb, m, A.cols - k - kb), A21, -1.0, 1.0, O_LOWER);
+    } else {
+      oview A12 = sub(A, k, k + kb, kb, A.cols - k - kb);
+      o_SolveTrm(A12, A11, 1.0, O_UPPER | O_TRANSA);
+      o_RankUpdateSym(sub(A, k + kb, k + kb, m, A.cols - k - kb), A12, -1.0, 1.0,
+                      O_UPPER | O_TRANSA);
+    }
+    k += kb;
+  }
+  return 0;
+}
+
+int o_DecomposeCHOL(oview A, int flags, int nb, int *pidx, double *pval) {
+  /* chol.go:116-127 */
+  if (A.cols != A.rows) return 1; /* "A not a square matrix" */
+  if (A.cols < nb || nb == 0) return unblockedCHOL(A, flags, 0, pidx, pval);
+  return blockedCHOL(A, flags, nb, pidx, pval);
+}
+
+void o_SolveCHOL(oview B, oview A, int flags) {
+  /* chol.go:147-158 */
+  if (flags & O_UPPER) {
+    o_SolveTrm(B, A, 1.0, O_UPPER | O_TRANSA);
+    o_SolveTrm(B, A, 1.0, O_UPPER);
+  } else if (flags & O_LOWER) {
+    o_SolveTrm(B, A, 1.0, O_LOWER);
+    o_SolveTrm(B, A, 1.0, O_LOWER | O_TRANSA);
+  }
+}
+
+/* ---- LU with partial pivoting, lu.go ---- */
+static void unblockedLUpiv(oview A, int64_t *p, int np) {
+  /* lu.go:103-170, FLAME LU variant 3 (left-looking); p relative to A's top */
+  int k = 0;
+  for (; k < A.rows && k < A.cols; k++) {
+    oview a1 = sub(A, 0, k, A.rows, 1);             /* current column */
+    oview A00 = sub(A, 0, 0, k, k);
+    oview a01 = sub(A, 0, k, k, 1);
+    oview a10 = sub(A, k, 0, 1, k);
+    oview a11 = sub(A, k, k, 1, 1);
+    oview A20 = sub(A, k + 1, 0, A.rows - k - 1, k);
+    oview a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    applyPivots(a1, p, k < np ? k : np);            /* :131 */
+    MVSolveTrm(a01, A00, O_LOWER | O_UNIT);          /* :134 */
+    a11.p[0] += Dot(a10, a01, -1.0);                /* :136 */
+    MVMult(a21, A20, a01, -1.0, 1.0, 0);            /* :138 */
+    oview aB1 = sub(A, k, k, A.rows - k, 1);
+    pivotIndex(aB1, &p[k]);                         /* :142 */
+    applyPivots(aB1, &p[k], 1);                     /* :145 */
+    InvScale(a21, a11.p[0]);                        /* :148 */
+    applyPivots(sub(A, k, 0, A.rows - k, k), &p[k], 1); /* :151-152 */
+    p[k] += k;                                      /* :154 */
+  }
+  if (k < A.cols) { /* :165-168 */
+    oview ATR = sub(A, 0, k, k, A.cols - k);
+    applyPivots(ATR, p, np);
+    o_SolveTrm(ATR, sub(A, 0, 0, k, k), 1.0, O_LEFT | O_UNIT | O_LOWER);
+  }
+}
+
+static void blockedLUpiv(oview A, int64_t *p, int np, int nb) {
+  /* lu.go:173-236 */
+  int k = 0;
+  while (k < A.rows && k < A.cols) {
+    int kb = k + nb > A.cols ? A.cols - k : nb;    /* partition.go:252-254 */
+    int pkb = k + kb > np ? np - k : kb;           /* partition.go:405-407 */
+    oview A1 = sub(A, 0, k, A.rows, kb);
+    applyPivots(A1, p, k);                          /* :201 */
+    oview A00 = sub(A, 0, 0, k, k);
+    oview A01 = sub(A, 0, k, k, kb);
+    o_SolveTrm(A01, A00, 1.0, O_LOWER | O_UNIT);    /* :204 */
+    o_Mult(sub(A, k, k, kb, kb), sub(A, k, 0, kb, k), A01, -1.0, 1.0, 0);        /* :206 */
+    int m2 = A.rows - k - kb;
+    o_Mult(sub(A, k + kb, k, m2 > 0 ? m2 : 0, kb), sub(A, k + kb, 0, m2 > 0 ? m2 : 0, k), A01,
+           -1.0, 1.0, 0);                           /* :208 */
+    oview AB1 = sub(A, k, k, A.rows - k, kb);
+    unblockedLUpiv(AB1, &p[k], pkb);                /* :212 */
+    applyPivots(sub(A, k, 0, A.rows - k, k), &p[k], pkb); /* :216 */
+    for (int i = 0; i < pkb; i++) p[k + i] += k;    /* :218-220 */
+    k += kb;
+  }
+  if (k < A.cols) { /* :231-234 */
+    oview ATR = sub(A, 0, k, k, A.cols - k);
+    applyPivots(ATR, p, np);
+    o_SolveTrm(ATR, sub(A, 0, 0, k, k), 1.0, O_LEFT | O_UNIT | O_LOWER);
+  }
+}
+
+int o_DecomposeLU(oview A, int64_t *pivots, int npiv, int nb) {
+  /* lu.go:257-273 */
+  int mlen = A.rows < A.cols ? A.rows : A.cols;
+  if (npiv < mlen) return 1; /* "pivot array < min(A.Rows(),A.Cols())" */
+  memset(pivots, 0, sizeof(int64_t) * (size_t)npiv);
+  if (mlen <= nb || nb == 0)
+    unblockedLUpiv(A, pivots, npiv);
+  else
+    blockedLUpiv(A, pivots, npiv, nb);
+  return 0;
+}
+
+int o_SolveLU(oview B, oview A, const int64_t *pivots, int npiv, int flags) {
+  /* lu.go:320-334, including its TRANSA branch as written (pivots first,
+   * then L^T and U^T) */
+  applyPivots(B, pivots, npiv);
+  if (flags & O_TRANSA) {
+    o_SolveTrm(B, A, 1.0, O_LOWER | O_UNIT | O_TRANSA);
+    o_SolveTrm(B, A, 1.0, O_UPPER | O_TRANSA);
+  } else {
+    o_SolveTrm(B, A, 1.0, O_LOWER | O_UNIT);
+    o_SolveTrm(B, A, 1.0, O_UPPER);
+  }
+  return 0;
+}
+
+/* ---- reference Cholesky adjoint (no counterpart in matops) ----
+ * Abar = 1/2 (S + S^T), S = L^-T Phi(L^T Lbar) L^-1 (the closed form of the
+ * Cholesky derivative, e.g. Murray 2016, arXiv:1602.07527, eq. 10), computed
+ * with the oracle's own GEMM/TRSM. */
+void o_chol_adjoint(int n, const double *L, int ldl, const double *Lbar, int ldb, double *Abar,
+                    int lda) {
+  double *Lt = calloc((size_t)n * n, sizeof(double));
+  double *Lb = calloc((size_t)n * n, sizeof(double));
+  double *P = calloc((size_t)n * n, sizeof(double));
+  for (int j = 0; j < n; j++)
+    for (int i = j; i < n; i++) {
+      Lt[i + (size_t)j * n] = L[i + (size_t)j * ldl];
+      Lb[i + (size_t)j * n] = Lbar[i + (size_t)j * ldb];
+    }
+  oview vL = {Lt, n, n, n}, vB = {Lb, n, n, n}, vP = {P, n, n, n};
+  o_Mult(vP, vL, vB, 1.0, 0.0, O_TRANSA); /* P = L^T Lbar */
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < j; i++) P[i + (size_t)j * n] = 0.0;
+    P[j + (size_t)j * n] *= 0.5;
+  }
+  o_SolveTrm(vP, vL, 1.0, O_LEFT | O_LOWER | O_TRANSA); /* L^-T P */
+  o_SolveTrm(vP, vL, 1.0, O_RIGHT | O_LOWER);           /* ... L^-1 */
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++)
+      Abar[i + (size_t)j * lda] = 0.5 * (P[i + (size_t)j * n] + P[j + (size_t)i * n]);
+  free(Lt);
+  free(Lb);
+  free(P);
+}

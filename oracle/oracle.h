@@ -56,6 +56,7 @@ extern const okern_t o_port_kernels;
  * Returns 0 on success, -1 when the shared object or a symbol is missing. */
 int o_use_kernels(int which, const char *path);
 int o_kernels_in_use(void);
+int o_use_kernel_mask(int mask, const char *path);
 
 /* tuning globals of mmult.go:33-44 */
 void o_BlockingParams(int mb, int nb, int kb);
