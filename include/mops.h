@@ -134,6 +134,36 @@ int mops_d_dpotrf(int flags, int N, int nb, double *A, int lda, int *info, void 
 int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
                   void *stream);
 
+/* ---- LU with partial pivoting -------------------------------------------- */
+
+/* Replaces DecomposeLU / blockedLUpiv / unblockedLUpiv (lu.go:103-273) with
+ * pivotIndex / applyPivots (pivot.go:55-61, 98-107): A (M x N) = P^T L U in
+ * place, unit L below the diagonal, U on and above it. pivots: npiv int64
+ * entries (Go []int); all are cleared, then the first min(M,N) receive the
+ * 0-based global pivot rows (p[k] >= k, p[k] == k: no interchange), equal to
+ * LAPACK ipiv - 1. Ties between |candidates| go to the lowest row. A zero
+ * pivot is not an error (Inf/NaN propagate, as in the reference). Returns
+ * -3 with "pivot array < min(A.Rows(),A.Cols())" when npiv < min(M,N).
+ * Host memory, synchronous. */
+int mops_dgetrf(mops_mdata_t *A, int64_t *pivots, int npiv, int M, int N, int nb);
+
+/* Replaces SolveLU (lu.go:320-334): B (N x nrhs) = A^-1 B from the factors
+ * and pivots of mops_dgetrf; TRANSA solves A^T X = B (correctly; the
+ * reference applies the pivots first on that branch, which does not).
+ * Host memory, synchronous. */
+int mops_dgetrs(mops_mdata_t *B, const mops_mdata_t *A, const int64_t *pivots, int npiv,
+                int flags, int N, int nrhs);
+
+/* Device-resident twins. ipiv: device int64[min(M,N)] (may be NULL); perm:
+ * device int[M] (may be NULL) receiving the row permutation (perm[i] =
+ * original row now at i); status: device int (may be NULL), 0 on success,
+ * 1 if a panel's grid barrier timed out. mops_d_dgetrs uses perm when given,
+ * else rebuilds it from the npiv pivots. */
+int mops_d_dgetrf(int M, int N, int nb, double *A, int lda, int64_t *ipiv, int *perm,
+                  int *status, void *stream);
+int mops_d_dgetrs(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,
+                  int npiv, const int *perm, double *B, int ldb, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,15 +37,27 @@ SIGNATURES = {
     "mops_dpotrs": (_I, [_PM, _PM, _I, _I, _I]),
     "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),
     "mops_d_dpotrs": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+    "mops_dgetrf": (_I, [_PM, _P, _I, _I, _I, _I]),
+    "mops_dgetrs": (_I, [_PM, _PM, _P, _I, _I, _I, _I]),
+    "mops_d_dgetrf": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P]),
+    "mops_d_dgetrs": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P]),
 }
 
 _lib = None
 
 
 def lib():
-    """Loads libmops.so once and declares every signature."""
+    """Loads libmops.so once and declares every signature.
+
+    When torch is importable it is imported first: torch ships its own HIP
+    runtime, and loading it before libmops makes both share that one runtime
+    instead of two copies competing for the device."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIBPATH):
             raise RuntimeError(
                 f"{LIBPATH} is not built; run `python -m matops_amd.build` first")

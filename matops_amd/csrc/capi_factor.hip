@@ -110,3 +110,103 @@ int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nr
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting
+
+#include "mops_lu.hpp"
+
+extern "C" {
+
+int mops_d_dgetrf(int M, int N, int nb, double* A, int lda, int64_t* ipiv, int* perm,
+                  int* status, void* stream) {
+  clear_error();
+  if (M < 0) return -1;
+  if (N < 0) return -2;
+  if (nb < 0) return -3;
+  if (lda < (M > 1 ? M : 1)) return -5;
+  if (M == 0 || N == 0) return ST_OK;
+  MOPS_TRY(ws_reserve(getrf_ws_bytes(M, N, nb)));
+  return getrf_d(as_stream(stream), M, N, nb, A, lda, reinterpret_cast<long long*>(ipiv), perm,
+                 status);
+}
+
+int mops_d_dgetrs(int flags, int N, int nrhs, const double* A, int lda, const int64_t* ipiv,
+                  int npiv, const int* perm, double* B, int ldb, void* stream) {
+  clear_error();
+  if (N < 0) return -2;
+  if (nrhs < 0) return -3;
+  if (lda < (N > 1 ? N : 1)) return -5;
+  if (!perm && !ipiv) return -6;
+  if (ldb < (N > 1 ? N : 1)) return -10;
+  if (N == 0 || nrhs == 0) return ST_OK;
+  MOPS_TRY(ws_reserve(getrs_ws_bytes(N, nrhs)));
+  return getrs_d(as_stream(stream), flags, N, nrhs, A, lda,
+                 reinterpret_cast<const long long*>(ipiv), npiv, perm, B, ldb);
+}
+
+int mops_dgetrf(mops_mdata_t* A, int64_t* pivots, int npiv, int M, int N, int nb) {
+  clear_error();
+  if (!A || (M > 0 && N > 0 && !A->md)) return -1;
+  const int mn = M < N ? M : N;
+  if (npiv < mn) {
+    set_error("pivot array < min(A.Rows(),A.Cols())");  // lu.go:260-262
+    return -3;
+  }
+  if (M < 0) return -4;
+  if (N < 0) return -5;
+  if (nb < 0) return -6;
+  for (int k = 0; k < npiv; ++k) pivots[k] = 0;  // lu.go:264-266
+  if (M == 0 || N == 0) return ST_OK;
+  if (A->step < M) return -1;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)M * N;
+  MOPS_TRY(ws_reserve(na * sizeof(double) + (size_t)mn * 8 + getrf_ws_bytes(M, N, nb) + 4096));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  long long* dp = (long long*)ws_carve((size_t)mn * sizeof(long long));
+  int* dst = (int*)ws_carve(sizeof(int));
+  MOPS_TRY(h2d(dA, M, A->md, A->step, M, N, s));
+  MOPS_TRY(getrf_d(s, M, N, nb, dA, M, dp, nullptr, dst));
+  MOPS_TRY(d2h(A->md, A->step, dA, M, M, N, s));
+  int st = 0;
+  MOPS_CHECK_HIP(hipMemcpyAsync(pivots, dp, (size_t)mn * sizeof(long long),
+                                hipMemcpyDeviceToHost, s));
+  MOPS_CHECK_HIP(hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  if (st != 0) {
+    set_error("LU panel: grid barrier timed out");
+    return ST_HIP;
+  }
+  return ST_OK;
+}
+
+int mops_dgetrs(mops_mdata_t* B, const mops_mdata_t* A, const int64_t* pivots, int npiv,
+                int flags, int N, int nrhs) {
+  clear_error();
+  if (!B || (N > 0 && nrhs > 0 && !B->md)) return -1;
+  if (!A || (N > 0 && !A->md)) return -2;
+  if (N < 0) return -6;
+  if (nrhs < 0) return -7;
+  if (N == 0 || nrhs == 0) return ST_OK;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  const int np = npiv < N ? npiv : N;
+  MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + (size_t)np * 8 + getrs_ws_bytes(N, nrhs) +
+                      4096));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  double* dB = (double*)ws_carve(nb * sizeof(double));
+  long long* dp = (long long*)ws_carve((size_t)(np > 0 ? np : 1) * sizeof(long long));
+  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
+  MOPS_TRY(h2d(dB, N, B->md, B->step, N, nrhs, s));
+  if (np > 0)
+    MOPS_CHECK_HIP(hipMemcpyAsync(dp, pivots, (size_t)np * sizeof(long long),
+                                  hipMemcpyHostToDevice, s));
+  MOPS_TRY(getrs_d(s, flags, N, nrhs, dA, N, dp, np, nullptr, dB, N));
+  MOPS_TRY(d2h(B->md, B->step, dB, N, N, nrhs, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
+}  // extern "C"

@@ -178,6 +178,52 @@ size_t potrf_ws_bytes(int N, int nb) {
   return x + t + 1024;
 }
 
+namespace {
+
+// Diagonal block k (order kb) and the panel below / right of it:
+// factor + inverse, then A21 = A21 * inv(L11)^T (or A12 = inv(U11)^T * A12).
+int chol_panel(hipStream_t s, bool upper, int N, int k, int kb, double* A, long lda, double* X,
+               double* T, int* info, double* badval) {
+  double* Akk = A + k + (long)k * lda;
+  MOPS_TRY(chol_diag(s, upper, kb, Akk, lda, X, kb, k, info, badval, T));
+  const int m = N - k - kb;
+  if (m <= 0) return ST_OK;
+  if (!upper) {
+    double* A21 = A + (k + kb) + (long)k * lda;
+    MOPS_TRY(copy_tile_d(s, m, kb, A21, lda, T, m));
+    return gemm_d(s, false, true, m, kb, kb, 1.0, T, m, X, kb, 0.0, A21, lda, GM_FULL,
+                  TRI_B_UPPER);
+  }
+  double* A12 = A + k + (long)(k + kb) * lda;
+  MOPS_TRY(copy_tile_d(s, kb, m, A12, lda, T, kb));
+  return gemm_d(s, true, false, kb, m, kb, 1.0, X, kb, T, kb, 0.0, A12, lda, GM_FULL,
+                TRI_A_LOWER);
+}
+
+// Update of the trailing block that starts c0 columns after the panel's
+// end (rows and columns j0 = k+kb+c0 ... j0+nc of the trailing matrix),
+// with the panel's rows/columns r0 = k+kb+c0 ... : trap = only the next
+// block column (a trapezoid of nc columns), else the square to the end.
+int chol_update(hipStream_t s, bool upper, int N, int k, int kb, int c0, int nc, bool trap,
+                double* A, long lda) {
+  const int j0 = k + kb + c0;
+  const int rows = N - j0;
+  if (rows <= 0 || nc <= 0) return ST_OK;
+  double* C = A + j0 + (long)j0 * lda;
+  if (!upper) {
+    const double* P = A + j0 + (long)k * lda;  // A21 rows j0..N
+    if (trap)
+      return gemm_d(s, false, true, rows, nc, kb, -1.0, P, lda, P, lda, 1.0, C, lda, GM_LOWER_T);
+    return gemm_d(s, false, true, rows, rows, kb, -1.0, P, lda, P, lda, 1.0, C, lda, GM_LOWER);
+  }
+  const double* P = A + k + (long)j0 * lda;  // A12 columns j0..N
+  if (trap)
+    return gemm_d(s, true, false, nc, rows, kb, -1.0, P, lda, P, lda, 1.0, C, lda, GM_UPPER_T);
+  return gemm_d(s, true, false, rows, rows, kb, -1.0, P, lda, P, lda, 1.0, C, lda, GM_UPPER);
+}
+
+}  // namespace
+
 int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* info,
             double* badval) {
   if (N <= 0) return ST_OK;
@@ -196,32 +242,57 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   }
   hipLaunchKernelGGL(k_init_info, dim3(1), dim3(1), 0, s, info, badval);
   MOPS_CHECK_HIP(hipGetLastError());
-  for (int k = 0; k < N; k += nb) {
-    const int kb = min(nb, N - k);
-    double* Akk = A + k + (long)k * lda;
-    MOPS_TRY(chol_diag(s, upper, kb, Akk, lda, X, kb, k, info, badval, T));
-    const int m = N - k - kb;
-    if (m <= 0) break;
-    double* A22 = A + (k + kb) + (long)(k + kb) * lda;
-    if (!upper) {
-      double* A21 = A + (k + kb) + (long)k * lda;
-      MOPS_TRY(copy_tile_d(s, m, kb, A21, lda, T, m));
-      MOPS_TRY(gemm_d(s, false, true, m, kb, kb, 1.0, T, m, X, kb, 0.0, A21, lda, GM_FULL,
-                      TRI_B_UPPER));
+  const int nsteps = ceil_div(N, nb);
+  if (nsteps <= 2) {
+    for (int k = 0; k < N; k += nb) {
+      const int kb = min(nb, N - k);
+      MOPS_TRY(chol_panel(s, upper, N, k, kb, A, lda, X, T, info, badval));
+      const int m = N - k - kb;
+      if (m <= 0) break;
       trace_mark_begin(s, TR_SYRK_UPDATE);
-      MOPS_TRY(gemm_d(s, false, true, m, m, kb, -1.0, A21, lda, A21, lda, 1.0, A22, lda,
-                      GM_LOWER));
-      trace_mark_end(s, TR_SYRK_UPDATE, (double)m * (m + 1) * kb);
-    } else {
-      double* A12 = A + k + (long)(k + kb) * lda;
-      MOPS_TRY(copy_tile_d(s, kb, m, A12, lda, T, kb));
-      MOPS_TRY(gemm_d(s, true, false, kb, m, kb, 1.0, X, kb, T, kb, 0.0, A12, lda, GM_FULL,
-                      TRI_A_LOWER));
-      trace_mark_begin(s, TR_SYRK_UPDATE);
-      MOPS_TRY(gemm_d(s, true, false, m, m, kb, -1.0, A12, lda, A12, lda, 1.0, A22, lda,
-                      GM_UPPER));
+      MOPS_TRY(chol_update(s, upper, N, k, kb, 0, m, false, A, lda));
       trace_mark_end(s, TR_SYRK_UPDATE, (double)m * (m + 1) * kb);
     }
+  } else {
+    // Look-ahead: the panel of step k+1 (and the update of its block column
+    // by step k) runs on a second stream while the caller's stream applies
+    // step k to the rest of the trailing matrix.
+    hipStream_t s2 = aux_stream();
+    hipEvent_t e_fork = sync_event(0), e_pan = sync_event(1), e_rest = sync_event(2),
+               e_join = sync_event(3);
+    if (!s2 || !e_fork || !e_pan || !e_rest || !e_join) {
+      set_error("potrf: could not create the look-ahead stream");
+      return ST_HIP;
+    }
+    MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
+    MOPS_TRY(chol_panel(s2, upper, N, 0, min(nb, N), A, lda, X, T, info, badval));
+    MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
+    bool have_rest = false;
+    for (int k = 0; k < N; k += nb) {
+      const int kb = min(nb, N - k);
+      const int m = N - k - kb;
+      if (m <= 0) break;
+      const int kb1 = min(nb, m);  // width of the next block column
+      // s2 may touch block column k+1 only after the previous rest update
+      if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
+      // caller's stream: rest of the trailing matrix, after panel k
+      MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
+      if (m > kb1) {
+        const int mr = m - kb1;
+        trace_mark_begin(s, TR_SYRK_UPDATE);
+        MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, mr, false, A, lda));
+        trace_mark_end(s, TR_SYRK_UPDATE, (double)mr * (mr + 1) * kb);
+        MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
+        have_rest = true;
+      }
+      // second stream: next block column, then the next panel
+      MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
+      MOPS_TRY(chol_panel(s2, upper, N, k + kb, kb1, A, lda, X, T, info, badval));
+      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
+    }
+    MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
   }
   hipLaunchKernelGGL(k_fix_info, dim3(1), dim3(1), 0, s, info);
   MOPS_CHECK_HIP(hipGetLastError());

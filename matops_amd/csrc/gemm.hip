@@ -82,7 +82,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int LDS_A = BM + 4, LDS_B = BN + 4;
+  // Large tiles: row pitch = 128 bytes modulo 256, so the two k-rows read by
+  // lanes 0-31 of one ds_read_b64 fall in disjoint bank halves. Small
+  // tiles keep a short pad so that two workgroups fit one CU's LDS.
+  constexpr int PAD = BM >= 128 ? 16 : 4;
+  constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
   constexpr int A_PER = BM * BK / NT, B_PER = BN * BK / NT;
   static_assert(A_PER * NT == BM * BK && B_PER * NT == BN * BK, "tile");
   typedef typename Mfma<T>::acc_t acc_t;
@@ -93,7 +97,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
   // ---- tile selection ----
   const int lin = xcd_linear(blockIdx.x, gridDim.x);
   int tm, tn;
-  if (g.mode == GM_FULL) {
+  if (g.mode == GM_FULL || g.mode == GM_LOWER_T || g.mode == GM_UPPER_T) {
     const int per_group = GROUP_M * g.tiles_n;
     const int grp = lin / per_group;
     const int first = grp * GROUP_M;
@@ -122,6 +126,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
     }
   }
   const int m0 = tm * BM, n0 = tn * BN;
+  // trapezoids: skip tiles entirely on the other side of the diagonal
+  if (g.mode == GM_LOWER_T && m0 + BM <= n0) return;
+  if (g.mode == GM_UPPER_T && m0 >= n0 + BN) return;
 
   int kbeg = 0, kend = g.K;
   switch (g.tri) {
@@ -256,8 +263,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
       for (int r = 0; r < 4; ++r) {
         const int gn = n0 + wn * WN + j * 16 + Mfma<T>::drow(lk, r);
         bool ok = gm < M && gn < N;
-        if (g.mode == GM_LOWER) ok = ok && gm >= gn;
-        if (g.mode == GM_UPPER) ok = ok && gm <= gn;
+        if (g.mode == GM_LOWER || g.mode == GM_LOWER_T) ok = ok && gm >= gn;
+        if (g.mode == GM_UPPER || g.mode == GM_UPPER_T) ok = ok && gm <= gn;
         if (ok) {
           T* p = C + gm + (long)gn * ldc;
           T v = alpha * acc[i][j][r];
@@ -302,7 +309,7 @@ int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int b
   g.tiles_m = ceil_div(g.M, BM);
   g.tiles_n = ceil_div(g.N, BN);
   long nblk;
-  if (g.mode == GM_FULL) {
+  if (g.mode == GM_FULL || g.mode == GM_LOWER_T || g.mode == GM_UPPER_T) {
     nblk = (long)g.tiles_m * g.tiles_n;
   } else {
     const long ts = ceil_div(g.tiles_m, SG);
@@ -340,7 +347,7 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
     set_error("gemm: batch too large");
     return ST_BADARG;
   }
-  if (mode != GM_FULL && M != N) {
+  if ((mode == GM_LOWER || mode == GM_UPPER) && M != N) {
     set_error("gemm: triangular update needs a square C");
     return ST_BADARG;
   }
@@ -352,9 +359,12 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
     }
     if (mode != GM_FULL) {
       // triangle-only scale: rare, fall back to per-column launches
+      const bool lo = mode == GM_LOWER || mode == GM_LOWER_T;
       for (int j = 0; j < N; ++j) {
-        int r0 = mode == GM_LOWER ? j : 0;
-        int rows = mode == GM_LOWER ? M - j : j + 1;
+        int r0 = lo ? j : 0;
+        int rows = lo ? M - j : j + 1;
+        if (lo && j >= M) break;
+        if (!lo && rows > M) rows = M;
         hipLaunchKernelGGL(k_scale_tile<T>, dim3(grid_for(rows)), dim3(256), 0, s, rows, 1,
                            beta, C + r0 + (long)j * ldc, ldc);
       }

@@ -74,11 +74,19 @@ struct Workspace {
 int ws_reserve(size_t bytes);
 void* ws_carve(size_t bytes);  // 256-byte aligned; nullptr when out of space
 void ws_reset();
+// Scoped carving: buffers carved after ws_mark() are given back by
+// ws_release(mark). Safe for work queued on one stream (it runs in order).
+size_t ws_mark();
+void ws_release(size_t mark);
 // Frees the calling thread's workspace and streams on every device.
 void thread_release();
 
 // Per-thread stream used by the host-pointer entry points.
 hipStream_t host_stream();
+// Per-thread second stream (default priority) for look-ahead work, and four
+// reusable synchronisation events (index 0..3) of the calling thread.
+hipStream_t aux_stream();
+hipEvent_t sync_event(int i);
 
 // Optional timing of selected launches with HIP events recorded on the
 // launching stream (per host thread). Classes name what is timed.

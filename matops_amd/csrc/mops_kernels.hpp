@@ -8,7 +8,9 @@ namespace mops {
 
 // Which part of C a GEMM writes. GM_LOWER / GM_UPPER require M == N and
 // only tiles that meet the triangle are launched (SYRK-shaped updates).
-enum : int { GM_FULL = 0, GM_LOWER = 1, GM_UPPER = 2 };
+// GM_LOWER_T / GM_UPPER_T: any M x N, only elements with row >= col
+// (row <= col) are written (trapezoids: a diagonal block plus a rectangle).
+enum : int { GM_FULL = 0, GM_LOWER = 1, GM_UPPER = 2, GM_LOWER_T = 3, GM_UPPER_T = 4 };
 
 // Known-zero structure of an operand, used only to skip k-slabs that are
 // multiplied by stored zeros. The zeros must really be stored: a skipped

@@ -18,6 +18,8 @@ thread_local char g_err[512] = {0};
 struct ThreadState {
   Workspace ws[16];
   hipStream_t stream[16] = {nullptr};
+  hipStream_t aux[16] = {nullptr};
+  hipEvent_t ev[16][4] = {};
 };
 thread_local ThreadState g_ts;
 
@@ -79,17 +81,32 @@ void* ws_carve(size_t bytes) {
 
 void ws_reset() { g_ts.ws[current_device()].used = 0; }
 
+size_t ws_mark() { return g_ts.ws[current_device()].used; }
+
+void ws_release(size_t mark) {
+  Workspace& w = g_ts.ws[current_device()];
+  if (mark <= w.used) w.used = mark;
+}
+
 void thread_release() {
   int cur = 0;
   (void)hipGetDevice(&cur);
   for (int d = 0; d < 16; ++d) {
-    if (!g_ts.ws[d].base && !g_ts.stream[d]) continue;
+    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d]) continue;
     if (hipSetDevice(d) != hipSuccess) continue;
     if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
     if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
     if (g_ts.stream[d]) (void)hipStreamDestroy(g_ts.stream[d]);
+    if (g_ts.aux[d]) {
+      (void)hipStreamSynchronize(g_ts.aux[d]);
+      (void)hipStreamDestroy(g_ts.aux[d]);
+    }
+    for (int e = 0; e < 4; ++e)
+      if (g_ts.ev[d][e]) (void)hipEventDestroy(g_ts.ev[d][e]);
     g_ts.ws[d] = Workspace();
     g_ts.stream[d] = nullptr;
+    g_ts.aux[d] = nullptr;
+    for (int e = 0; e < 4; ++e) g_ts.ev[d][e] = nullptr;
   }
   (void)hipSetDevice(cur);
 }
@@ -164,6 +181,27 @@ int trace_collect(int cls, int* count, double* ms, double* flops) {
   *ms = t;
   *flops = f;
   return ST_OK;
+}
+
+hipStream_t aux_stream() {
+  const int d = current_device();
+  if (!g_ts.aux[d]) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    g_ts.aux[d] = s;
+  }
+  return g_ts.aux[d];
+}
+
+hipEvent_t sync_event(int i) {
+  const int d = current_device();
+  if (i < 0 || i >= 4) return nullptr;
+  if (!g_ts.ev[d][i]) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    g_ts.ev[d][i] = e;
+  }
+  return g_ts.ev[d][i];
 }
 
 hipStream_t host_stream() {

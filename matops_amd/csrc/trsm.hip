@@ -71,6 +71,114 @@ __global__ __launch_bounds__(LEAF* LEAF_LPC) void k_trtri_leaf(int bb, int lower
   }
 }
 
+// ---- LEFT solve for few right-hand sides: one kernel, blocks chained by flags
+//
+// Workgroup (b, g) solves block row b (64 rows) of op(A) X = B for the
+// column group g (16 columns): acc = B_b - sum_c op(A)_bc X_c over the
+// already solved blocks c (waiting on their flags in solve order), then
+// X_b = op(inv(A_bb)) acc, written over B_b, and flag (b, g) is released.
+// Workgroups are numbered in solve order, so every block a workgroup waits
+// for belongs to a workgroup dispatched before it. The products run on the
+// FP64 matrix cores (v_mfma_f64_16x16x4, one 16x16 accumulator per wave).
+constexpr int SK_BS = 64, SK_CG = 16;
+constexpr unsigned SK_SPIN_LIMIT = 1u << 24;
+
+__global__ __launch_bounds__(256) void k_trsm_skinny(int N, int nrhs, int eff_lower, int trans,
+                                                     const double* __restrict__ A, long lda,
+                                                     const double* __restrict__ Dinv,
+                                                     double* B, long ldb, int* flags, int* err) {
+  __shared__ double At[SK_BS][SK_BS + 1];  // op(A) block [row][k]
+  __shared__ double Xs[SK_BS][SK_CG + 1];  // solved block / accumulator [k][col]
+  const int nblk = (N + SK_BS - 1) / SK_BS;
+  const int ngrp = (nrhs + SK_CG - 1) / SK_CG;
+  const int step = blockIdx.x / ngrp, g = blockIdx.x % ngrp;
+  const int b = eff_lower ? step : nblk - 1 - step;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = b * SK_BS, c0 = g * SK_CG;
+
+  // acc = B_b in the MFMA result layout: reg r <-> row 16w + lk + 4r, col lr
+  v4d acc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = r0 + 16 * w + lk + 4 * r, col = c0 + lr;
+    acc[r] = (row < N && col < nrhs) ? B[row + (long)col * ldb] : 0.0;
+  }
+  // op(A)[i][k] of block (bi, bk), staged in At[i][k]
+  auto load_a = [&](const double* base, long ld, int bi, int bk, int ni, int nk, bool tr) {
+    for (int idx = t; idx < SK_BS * SK_BS; idx += 256) {
+      const int x = idx % SK_BS, y = idx / SK_BS;  // x fastest: contiguous in memory
+      double v = 0.0;
+      if (!tr) {  // element (i = x, k = y) at base[bi*64 + x + (bk*64 + y) * ld]
+        if (x < ni && y < nk) v = base[(long)bi * SK_BS + x + ((long)bk * SK_BS + y) * ld];
+        At[x][y] = v;
+      } else {  // op(A)[i][k] = A[k][i]: element (k = x, i = y)
+        if (y < ni && x < nk) v = base[(long)bk * SK_BS + x + ((long)bi * SK_BS + y) * ld];
+        At[y][x] = v;
+      }
+    }
+  };
+  const int nb_rows = min(SK_BS, N - r0);
+  for (int it = 0; it < step; ++it) {
+    const int c = eff_lower ? it : nblk - 1 - it;
+    const int nc = min(SK_BS, N - c * SK_BS);
+    load_a(A, lda, b, c, nb_rows, nc, trans != 0);
+    if (t == 0) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&flags[c * ngrp + g], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+             0) {
+        if (++spins > SK_SPIN_LIMIT) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    for (int idx = t; idx < SK_BS * SK_CG; idx += 256) {
+      const int k = idx % SK_BS, col = idx / SK_BS;
+      const int gk = c * SK_BS + k, gc = c0 + col;
+      Xs[k][col] = (k < nc && gc < nrhs)
+                       ? __hip_atomic_load(&B[gk + (long)gc * ldb], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                       : 0.0;
+    }
+    __syncthreads();
+    v4d d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s4 = 0; s4 < SK_BS; s4 += 4)
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(At[16 * w + lr][s4 + lk], Xs[s4 + lk][lr], d, 0,
+                                               0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] -= d[r];
+    __syncthreads();
+  }
+  // X_b = op(Dinv_b) * acc ; Dinv_b is the 64x64 inverse of A_bb (ld 64)
+  const double* D = Dinv + (long)b * SK_BS * SK_BS;
+  load_a(D, SK_BS, 0, 0, nb_rows, nb_rows, trans != 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Xs[16 * w + lk + 4 * r][lr] = acc[r];
+  __syncthreads();
+  v4d x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s4 = 0; s4 < SK_BS; s4 += 4)
+    x = __builtin_amdgcn_mfma_f64_16x16x4f64(At[16 * w + lr][s4 + lk], Xs[s4 + lk][lr], x, 0, 0,
+                                             0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = r0 + 16 * w + lk + 4 * r, col = c0 + lr;
+    if (row < N && col < nrhs)
+      __hip_atomic_store(&B[row + (long)col * ldb], x[r], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 // All nblk blocks (order bb each, same strides) are inverted together.
 int trtri_core(hipStream_t s, bool lower, bool unit, int bb, int nblk, const double* A, long lda,
                long sA, double* X, long ldx, long sX, double* T) {
@@ -140,6 +248,8 @@ size_t trtri_core_ws(int bb, int nblk) {
   return (size_t)nblk * ((size_t)bb * bb / 2 + LEAF * LEAF) * sizeof(double);
 }
 
+constexpr int SKINNY_MAX_RHS = 128;
+
 inline int trsm_block(int N) {
   if (N <= 256) return N;
   return 512;
@@ -186,11 +296,28 @@ size_t trsm_ws_bytes(int flags, int N, int nrhs) {
   size_t inv = (size_t)nblk * bs * bs * sizeof(double);
   size_t tri = trtri_core_ws(bs, nblk);
   size_t w = (size_t)N * nrhs * sizeof(double);
-  return inv + tri + w + 4 * 256;
+  const int nb64 = ceil_div(N, 64);
+  size_t sk = (size_t)nb64 * 64 * 64 * sizeof(double) + (size_t)nb64 * 64 * sizeof(int) +
+              trtri_core_ws(64, nb64);
+  return (inv + tri + w > sk ? inv + tri + w : sk) + 6 * 256;
+}
+
+namespace {
+int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
+              double* B, long ldb);
 }
 
 int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
            double* B, long ldb) {
+  const size_t mark = ws_mark();
+  const int st = trsm_impl(s, flags, N, nrhs, alpha, A, lda, B, ldb);
+  ws_release(mark);
+  return st;
+}
+
+namespace {
+int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
+              double* B, long ldb) {
   if (N <= 0 || nrhs <= 0) return ST_OK;
   const bool right = flags & F_RIGHT;
   const bool upper = flags & F_UPPER;  // dmmat_solve_blk tests UPPER first (:879-906)
@@ -200,6 +327,23 @@ int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double
   if (alpha != 1.0) MOPS_TRY(scale_tile_d(s, right ? nrhs : N, right ? N : nrhs, alpha, B, ldb));
   if (alpha == 0.0) return ST_OK;
 
+  if (!right && nrhs <= SKINNY_MAX_RHS && N > SK_BS) {
+    const int nb64 = ceil_div(N, SK_BS), ngrp = ceil_div(nrhs, SK_CG);
+    double* D = (double*)ws_carve((size_t)nb64 * SK_BS * SK_BS * sizeof(double));
+    int* fl = (int*)ws_carve((size_t)(nb64 * ngrp + 1) * sizeof(int));
+    if (!D || !fl) {
+      set_error("trsm: workspace too small");
+      return ST_NOMEM;
+    }
+    MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, SK_BS, A, lda, D));
+    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nb64 * ngrp + 1, 256)), dim3(256), 0, s, fl,
+                       nb64 * ngrp + 1);
+    hipLaunchKernelGGL(k_trsm_skinny, dim3(nb64 * ngrp), dim3(256), 0, s, N, nrhs,
+                       eff_lower ? 1 : 0, trans ? 1 : 0, A, lda, D, B, ldb, fl,
+                       fl + nb64 * ngrp);
+    MOPS_CHECK_HIP(hipGetLastError());
+    return ST_OK;
+  }
   const int bs = trsm_block(N);
   const int nblk = ceil_div(N, bs);
   double* Dinv = (double*)ws_carve((size_t)nblk * bs * bs * sizeof(double));
@@ -256,5 +400,6 @@ int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double
   }
   return copy_tile_d(s, wrows, right ? N : nrhs, W, wrows, B, ldb);
 }
+}  // namespace
 
 }  // namespace mops

@@ -183,3 +183,35 @@ def SolveCHOL(B, A, flags):
     Am, _, _ = _view(A)
     check("mops_dpotrs", _lib.lib().mops_dpotrs(
         ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
+
+
+def DecomposeLU(A, pivots, nb):
+    """LU with partial pivoting in place (lu.go:257-273). pivots: int64
+    numpy array (Go []int), at least min(rows, cols) long. Returns (A, err)."""
+    mlen = min(A.shape)
+    if len(pivots) < mlen:
+        return A, MatopsError("pivot array < min(A.Rows(),A.Cols())")
+    if pivots.dtype != np.int64 or not pivots.flags.c_contiguous:
+        raise TypeError("pivots must be a contiguous int64 array")
+    pivots[:] = 0
+    if mlen == 0:
+        return A, None
+    Am, _, _ = _view(A)
+    check("mops_dgetrf", _lib.lib().mops_dgetrf(
+        ctypes.byref(Am), ctypes.c_void_p(pivots.ctypes.data), len(pivots),
+        A.shape[0], A.shape[1], nb))
+    return A, None
+
+
+def SolveLU(B, A, pivots, flags):
+    """B = A^-1 B (or A^-T B with TRANSA) from DecomposeLU (lu.go:320-334)."""
+    n = A.shape[0]
+    if n == 0 or B.shape[1] == 0:
+        return None
+    piv = np.ascontiguousarray(pivots, dtype=np.int64)
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dgetrs", _lib.lib().mops_dgetrs(
+        ctypes.byref(Bm), ctypes.byref(Am), ctypes.c_void_p(piv.ctypes.data),
+        len(piv), flags, n, B.shape[1]))
+    return None
