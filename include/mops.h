@@ -128,8 +128,8 @@ int mops_dpotrf(mops_mdata_t *A, int flags, int N, int nb);
 int mops_dpotrs(mops_mdata_t *B, const mops_mdata_t *A, int flags, int N, int nrhs);
 
 /* Device-resident twins. info: device int, set to -1 when the factorization
- * succeeded, else the 0-based index of the first NaN square root (may be
- * NULL). */
+ * succeeded, -2 when an internal wait timed out, else the 0-based index of
+ * the first NaN square root (may be NULL). */
 int mops_d_dpotrf(int flags, int N, int nb, double *A, int lda, int *info, void *stream);
 int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
                   void *stream);

@@ -80,6 +80,10 @@ int mops_dpotrf(mops_mdata_t* A, int flags, int N, int nb) {
   MOPS_CHECK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipMemcpyAsync(&bad, dbad, sizeof(double), hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  if (info == -2) {
+    set_error("Cholesky leaf: wait for the diagonal block timed out");
+    return ST_HIP;
+  }
   if (info >= 0) {
     // the panic text of unblockedCHOL (chol.go:35)
     set_error("illegal value at %d: %e", info, bad);

@@ -7,14 +7,14 @@
 //   A21 = A21 * tril(A11)^-T          (SolveTrm RIGHT|LOWER|TRANSA, :79)
 //   A22 = A22 - A21 * A21^T           (RankUpdateSym LOWER, :81)
 // and the UPPER variant with A12 / U11^-T (:82-87). The same step runs here
-// with the diagonal block factored recursively: leaves of up to 64x64 are
-// factored in registers by one workgroup (the reference's column algorithm)
-// which also returns the leaf's inverse; larger diagonal blocks are split in two and
-// joined with GEMMs, building inv(L11) on the way so that the panel solve is
-// one GEMM A21 = A21 * inv(L11)^T.
+// with the panel (diagonal block and the rows below it) factored
+// recursively: 64-column leaves factor their diagonal block in registers
+// (the reference's column algorithm), which also yields the leaf's inverse,
+// and solve the rows below with one GEMM; the halves are joined by a
+// trapezoidal GEMM update. The trailing update of step k runs on the
+// caller's stream while the next panel is factored on a second stream.
 #include <cmath>
 
-#include "leaf.hpp"
 #include "mops_solvers.hpp"
 
 namespace mops {
@@ -22,86 +22,133 @@ namespace {
 
 constexpr int LEAF = 64;
 
-// Cholesky of an n x n (n <= NL) diagonal block plus the inverse of the
-// factor, one workgroup of NL*LPC threads, the block held in registers
-// (layout in leaf.hpp). Per column j: the LPC lanes owning column j take the
-// square root (NaN -> info, chol.go:33-36), divide the rest of the column
-// by it (InvScale, chol.go:42) and publish it through LDS; after one barrier
-// every lane applies the rank-1 update to its part of the trailing
-// triangle (the SYR of chol.go:44). The UPPER case factors U^T = L.
-template <int NL, int LPC>
-__global__ __launch_bounds__(NL* LPC) void k_potf2_leaf(int n, int upper, double* A, long lda,
-                                                        double* X, long ldx, int goff, int* info,
-                                                        double* badval) {
-  constexpr int RPL = NL / LPC;
-  constexpr int LDL = NL + 1;
-  __shared__ double Ls[NL * LDL];
-  __shared__ double col[2][NL];
-  const int t = threadIdx.x;
-  const int c = t / LPC, sub = t % LPC;
-  const int lane = t & 63, base = lane & ~(LPC - 1);
-  double v[RPL];
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+constexpr unsigned LEAF_SPIN_LIMIT = 1u << 26;
+
+// One 64-column leaf of the Cholesky panel in one launch (64-thread
+// workgroups, one wave each).
+//  * workgroup 0 factors the n x n diagonal block (n <= 64): lane i holds row
+//    i in registers; column j takes the square root of the diagonal (a NaN
+//    records its global index, chol.go:33-36), scales the column by the
+//    reciprocal (InvScale, chol.go:42 -- built with -ffast-math, the
+//    reference's loop of divisions is a reciprocal multiply) and applies the
+//    rank-1 update to the rest of each row (the SYR of chol.go:44), the
+//    column entries broadcast with readlane. All loops are unrolled so the
+//    registers are addressed statically; the critical chain per column is a
+//    square root, a reciprocal and one multiply-add. The factor is stored
+//    and released to the other workgroups through a flag.
+//  * workgroup b >= 1 solves 64 rows below the block (X L^T = B, the TRSM of
+//    chol.go:79 for this leaf): lane = row, right-looking substitution with
+//    the factor broadcast from LDS.
+// The UPPER case runs the same on U^T (rows of U^T = columns of U).
+__global__ __launch_bounds__(64) void k_chol_leaf(int n, int mb, int upper, double* A, long lda,
+                                                  double* B, int goff, int* info,
+                                                  double* badval, int* flag, int* err) {
+  constexpr int NL = 64;
+  const int l = threadIdx.x;
+  if (blockIdx.x == 0) {
+    double a[NL];
 #pragma unroll
-  for (int q = 0; q < RPL; ++q) {
-    const int r = sub + LPC * q;
-    double a = 0.0;
-    if (r < n && c < n && r >= c) a = upper ? A[c + (long)r * lda] : A[r + (long)c * lda];
-    v[q] = a;
-  }
-  bool bad = false;
-  for (int j = 0; j < n; ++j) {
-    double* cj = col[j & 1];
-    if (c == j) {
-      const double d = __shfl(leaf::pick<RPL>(v, j / LPC), base | (j % LPC), 64);
-      const double r = sqrt(d);
-      if (isnan(r) && !bad && sub == j % LPC && info) {
-        bad = true;
-        const int old = atomicMin(info, goff + j);
-        if ((old < 0 || old > goff + j) && badval) *badval = d;
-      }
+    for (int c = 0; c < NL; ++c) {
+      double v = 0.0;
+      if (l < n && c < n && c <= l) v = upper ? A[c + (long)l * lda] : A[l + (long)c * lda];
+      a[c] = v;
+    }
 #pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        const int rr = sub + LPC * q;
-        if (rr > j) v[q] = v[q] / r;
-        if (rr == j) v[q] = r;
-        if (rr < NL) cj[rr] = rr >= j ? v[q] : 0.0;
+    for (int j = 0; j < NL; ++j) {
+      if (j < n) {
+        const double d = readlane_d(a[j], j);
+        const double r = sqrt(d);
+        if (l == 0 && isnan(r) && info) {
+          const int old = atomicMin(info, goff + j);
+          if ((old < 0 || old > goff + j) && badval) *badval = d;
+        }
+        const double ri = 1.0 / r;
+        a[j] = l > j ? a[j] * ri : (l == j ? r : a[j]);
+#pragma unroll
+        for (int k = j + 1; k < NL; ++k) {
+          if (k < n) {
+            const double lk = readlane_d(a[j], k);
+            if (l >= k) a[k] -= a[j] * lk;
+          }
+        }
       }
     }
-    __syncthreads();
-    if (c > j && c < n) {
-      const double lc = cj[c];
 #pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        const int rr = sub + LPC * q;
-        if (rr >= c && rr < n) v[q] -= cj[rr] * lc;
+    for (int c = 0; c < NL; ++c)
+      if (l < n && c < n && c <= l) {
+        if (upper)
+          A[c + (long)l * lda] = a[c];
+        else
+          A[l + (long)c * lda] = a[c];
       }
-    }
+    if (l == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return;
   }
-  // publish the factor: to global memory and to LDS for the inverse
+  // ---- rows below: row index of this lane
+  __shared__ double Ls[NL][NL + 1];
+  __shared__ double dinv[NL];
+  const int row = (blockIdx.x - 1) * NL + l;
+  double x[NL];
 #pragma unroll
-  for (int q = 0; q < RPL; ++q) {
-    const int r = sub + LPC * q;
-    if (c < NL) Ls[r * LDL + c] = v[q];
-    if (r < n && c < n && r >= c) {
-      if (upper)
-        A[c + (long)r * lda] = v[q];
-      else
-        A[r + (long)c * lda] = v[q];
+  for (int c = 0; c < NL; ++c) {
+    double v = 0.0;
+    if (row < mb && c < n) v = upper ? B[c + (long)row * lda] : B[row + (long)c * lda];
+    x[c] = v;
+  }
+  if (l == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (++spins > LEAF_SPIN_LIMIT) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
     }
   }
   __syncthreads();
-  double x[RPL];
-  leaf::inv_lower_column<NL, LPC, LDL>(Ls, n, c, sub, false, x);
+  // the factor, row by row: Ls[i][k] = L[i][k] (k <= i)
+  for (int idx = l; idx < NL * NL; idx += NL) {
+    const int i = idx % NL, k = idx / NL;
+    double v = 0.0;
+    if (i < n && k <= i)
+      v = upper ? __hip_atomic_load(&A[k + (long)i * lda], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT)
+                : __hip_atomic_load(&A[i + (long)k * lda], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    Ls[i][k] = v;
+  }
+  __syncthreads();
+  dinv[l] = l < n ? 1.0 / Ls[l][l] : 0.0;
+  __syncthreads();
 #pragma unroll
-  for (int q = 0; q < RPL; ++q) {
-    const int r = sub + LPC * q;
-    if (r < n && c < n) {
-      if (upper)
-        X[c + (long)r * ldx] = x[q];  // inv(U) = inv(L)^T, zeros below
-      else
-        X[r + (long)c * ldx] = x[q];  // inv(L), zeros above
+  for (int k = 0; k < NL; ++k) {
+    if (k < n) {
+      x[k] *= dinv[k];
+#pragma unroll
+      for (int c = k + 1; c < NL; ++c)
+        if (c < n) x[c] -= x[k] * Ls[c][k];
     }
   }
+  if (row < mb) {
+#pragma unroll
+    for (int c = 0; c < NL; ++c)
+      if (c < n) {
+        if (upper)
+          B[c + (long)row * lda] = x[c];
+        else
+          B[row + (long)c * lda] = x[c];
+      }
+  }
+}
+
+__global__ void k_zero_flags(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 __global__ void k_init_info(int* info, double* badval) {
@@ -109,95 +156,68 @@ __global__ void k_init_info(int* info, double* badval) {
   if (badval) *badval = 0.0;
 }
 
-__global__ void k_fix_info(int* info) {
-  if (info && *info == 0x7fffffff) *info = -1;
+// info: -1 when the factorization succeeded, -2 when a leaf's wait for its
+// diagonal block timed out, else the first NaN square root's index.
+__global__ void k_fix_info(int* info, const int* err) {
+  if (!info) return;
+  if (*err) *info = -2;
+  else if (*info == 0x7fffffff) *info = -1;
 }
 
-inline int split_point(int n) {
-  // first half: a multiple of the leaf size, about half of n
-  int h = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
-  if (h >= n) h = n - LEAF;
-  return h;
-}
+// Factor the panel of columns [j0, j0+w) (rows j0..N-1) in place: the
+// recursive (left half, trapezoidal update of the right half, right half)
+// scheme of a Cholesky panel, with 64-column leaves (k_chol_leaf: the
+// diagonal block and the rows below it in one launch). The UPPER case is
+// the transpose of each step.
+struct LeafFlags {
+  int* flags;  // one zeroed flag per leaf launch
+  int next;
+  int* err;
+};
 
-// Factor the n x n diagonal block at A (global offset goff) and write the
-// dense inverse of its factor into X (n x n, ld ldx). T: scratch of n*n.
-int chol_diag(hipStream_t s, bool upper, int n, double* A, long lda, double* X, long ldx, int goff,
-              int* info, double* badval, double* T) {
-  if (n <= LEAF) {
-    hipLaunchKernelGGL((k_potf2_leaf<64, 4>), dim3(1), dim3(256), 0, s, n, upper ? 1 : 0, A, lda,
-                       X, ldx, goff, info, badval);
+int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
+             int* info, double* badval) {
+  if (w <= LEAF) {
+    double* D = A + j0 + (long)j0 * lda;
+    const int m = N - j0 - w;
+    double* B = upper ? A + j0 + (long)(j0 + w) * lda : A + (j0 + w) + (long)j0 * lda;
+    int* flag = fl.flags + fl.next;
+    fl.next += 1;
+    hipLaunchKernelGGL(k_chol_leaf, dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(LEAF), 0, s, w,
+                       m, upper ? 1 : 0, D, lda, B, j0, info, badval, flag, fl.err);
     MOPS_CHECK_HIP(hipGetLastError());
     return ST_OK;
   }
-  const int h = split_point(n), m = n - h;
-  double* X11 = X;
-  double* X22 = X + h + (long)h * ldx;
-  MOPS_TRY(chol_diag(s, upper, h, A, lda, X11, ldx, goff, info, badval, T));
+  int h = ((w / 2 + LEAF - 1) / LEAF) * LEAF;
+  if (h >= w) h = w - LEAF;
+  MOPS_TRY(chol_rec(s, upper, N, j0, h, A, lda, fl, info, badval));
+  const int rows = N - j0 - h;
+  double* C = A + (j0 + h) + (long)(j0 + h) * lda;
   if (!upper) {
-    double* A21 = A + h;
-    double* A22 = A + h + (long)h * lda;
-    // A21 = A21 * X11^T ; A22 -= A21 A21^T
-    MOPS_TRY(copy_tile_d(s, m, h, A21, lda, T, m));
-    MOPS_TRY(gemm_d(s, false, true, m, h, h, 1.0, T, m, X11, ldx, 0.0, A21, lda, GM_FULL,
-                    TRI_B_UPPER));
-    MOPS_TRY(gemm_d(s, false, true, m, m, h, -1.0, A21, lda, A21, lda, 1.0, A22, lda, GM_LOWER));
-    MOPS_TRY(chol_diag(s, upper, m, A22, lda, X22, ldx, goff + h, info, badval, T));
-    // X21 = -X22 * A21 * X11 ; X12 = 0
-    MOPS_TRY(gemm_d(s, false, false, m, h, h, 1.0, A21, lda, X11, ldx, 0.0, T, m, GM_FULL,
-                    TRI_B_LOWER));
-    MOPS_TRY(gemm_d(s, false, false, m, h, m, -1.0, X22, ldx, T, m, 0.0, X + h, ldx, GM_FULL,
-                    TRI_A_LOWER));
-    MOPS_TRY(scale_tile_d(s, h, m, 0.0, X + (long)h * ldx, ldx));
+    const double* P = A + (j0 + h) + (long)j0 * lda;
+    MOPS_TRY(gemm_d(s, false, true, rows, w - h, h, -1.0, P, lda, P, lda, 1.0, C, lda,
+                    GM_LOWER_T));
   } else {
-    double* A12 = A + (long)h * lda;
-    double* A22 = A + h + (long)h * lda;
-    // A12 = X11^T * A12 ; A22 -= A12^T A12
-    MOPS_TRY(copy_tile_d(s, h, m, A12, lda, T, h));
-    MOPS_TRY(gemm_d(s, true, false, h, m, h, 1.0, X11, ldx, T, h, 0.0, A12, lda, GM_FULL,
-                    TRI_A_LOWER));
-    MOPS_TRY(gemm_d(s, true, false, m, m, h, -1.0, A12, lda, A12, lda, 1.0, A22, lda, GM_UPPER));
-    MOPS_TRY(chol_diag(s, upper, m, A22, lda, X22, ldx, goff + h, info, badval, T));
-    // X12 = -X11 * A12 * X22 ; X21 = 0
-    MOPS_TRY(gemm_d(s, false, false, h, m, m, 1.0, A12, lda, X22, ldx, 0.0, T, h, GM_FULL,
-                    TRI_B_UPPER));
-    MOPS_TRY(gemm_d(s, false, false, h, m, h, -1.0, X11, ldx, T, h, 0.0, X + (long)h * ldx, ldx,
-                    GM_FULL, TRI_A_UPPER));
-    MOPS_TRY(scale_tile_d(s, m, h, 0.0, X + h, ldx));
+    const double* P = A + j0 + (long)(j0 + h) * lda;
+    MOPS_TRY(gemm_d(s, true, false, w - h, rows, h, -1.0, P, lda, P, lda, 1.0, C, lda,
+                    GM_UPPER_T));
   }
-  return ST_OK;
+  return chol_rec(s, upper, N, j0 + h, w - h, A, lda, fl, info, badval);
 }
 
 }  // namespace
 
 size_t potrf_ws_bytes(int N, int nb) {
-  if (N <= 0) return 0;
   if (nb <= 0 || nb > N) nb = N;
-  const size_t x = (size_t)nb * nb * sizeof(double);
-  const size_t t = (size_t)N * nb * sizeof(double);  // panel copy, also >= nb*nb
-  return x + t + 1024;
+  return (size_t)(ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 8) * sizeof(int) + 1024;
 }
 
 namespace {
 
-// Diagonal block k (order kb) and the panel below / right of it:
-// factor + inverse, then A21 = A21 * inv(L11)^T (or A12 = inv(U11)^T * A12).
-int chol_panel(hipStream_t s, bool upper, int N, int k, int kb, double* A, long lda, double* X,
-               double* T, int* info, double* badval) {
-  double* Akk = A + k + (long)k * lda;
-  MOPS_TRY(chol_diag(s, upper, kb, Akk, lda, X, kb, k, info, badval, T));
-  const int m = N - k - kb;
-  if (m <= 0) return ST_OK;
-  if (!upper) {
-    double* A21 = A + (k + kb) + (long)k * lda;
-    MOPS_TRY(copy_tile_d(s, m, kb, A21, lda, T, m));
-    return gemm_d(s, false, true, m, kb, kb, 1.0, T, m, X, kb, 0.0, A21, lda, GM_FULL,
-                  TRI_B_UPPER);
-  }
-  double* A12 = A + k + (long)(k + kb) * lda;
-  MOPS_TRY(copy_tile_d(s, kb, m, A12, lda, T, kb));
-  return gemm_d(s, true, false, kb, m, kb, 1.0, X, kb, T, kb, 0.0, A12, lda, GM_FULL,
-                TRI_A_LOWER);
+// Panel k (diagonal block of order kb and everything below / right of it).
+int chol_panel(hipStream_t s, bool upper, int N, int k, int kb, double* A, long lda,
+               LeafFlags& fl, int* info, double* badval) {
+  return chol_rec(s, upper, N, k, kb, A, lda, fl, info, badval);
 }
 
 // Update of the trailing block that starts c0 columns after the panel's
@@ -234,19 +254,25 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   // unblocked and blocked algorithms compute the same factor; here both are
   // the blocked one with the outer block clamped to N.
   if (nb <= 0 || nb > N) nb = N;
-  double* X = (double*)ws_carve((size_t)nb * nb * sizeof(double));
-  double* T = (double*)ws_carve((size_t)N * nb * sizeof(double));
-  if (!X || !T) {
+  // one flag per 64-column leaf, plus an error word
+  const int nleaf = ceil_div(N, LEAF) + 2 * ceil_div(N, nb) + 4;
+  LeafFlags fl;
+  fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
+  fl.next = 0;
+  if (!fl.flags) {
     set_error("potrf: workspace too small");
     return ST_NOMEM;
   }
+  fl.err = fl.flags + nleaf;
+  hipLaunchKernelGGL(k_zero_flags, dim3(ceil_div(nleaf + 1, 256)), dim3(256), 0, s, fl.flags,
+                     nleaf + 1);
   hipLaunchKernelGGL(k_init_info, dim3(1), dim3(1), 0, s, info, badval);
   MOPS_CHECK_HIP(hipGetLastError());
   const int nsteps = ceil_div(N, nb);
   if (nsteps <= 2) {
     for (int k = 0; k < N; k += nb) {
       const int kb = min(nb, N - k);
-      MOPS_TRY(chol_panel(s, upper, N, k, kb, A, lda, X, T, info, badval));
+      MOPS_TRY(chol_panel(s, upper, N, k, kb, A, lda, fl, info, badval));
       const int m = N - k - kb;
       if (m <= 0) break;
       trace_mark_begin(s, TR_SYRK_UPDATE);
@@ -266,7 +292,7 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
     }
     MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
     MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
-    MOPS_TRY(chol_panel(s2, upper, N, 0, min(nb, N), A, lda, X, T, info, badval));
+    MOPS_TRY(chol_panel(s2, upper, N, 0, min(nb, N), A, lda, fl, info, badval));
     MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     bool have_rest = false;
     for (int k = 0; k < N; k += nb) {
@@ -288,13 +314,13 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       }
       // second stream: next block column, then the next panel
       MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
-      MOPS_TRY(chol_panel(s2, upper, N, k + kb, kb1, A, lda, X, T, info, badval));
+      MOPS_TRY(chol_panel(s2, upper, N, k + kb, kb1, A, lda, fl, info, badval));
       MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     }
     MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
     MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
   }
-  hipLaunchKernelGGL(k_fix_info, dim3(1), dim3(1), 0, s, info);
+  hipLaunchKernelGGL(k_fix_info, dim3(1), dim3(1), 0, s, info, fl.err);
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }

@@ -1,12 +1,13 @@
-// Register-resident kernels for one small diagonal block (NL <= 128):
-// Cholesky of the block and inverse of a triangular factor.
+// Register-resident kernels for one 64x64 diagonal block: Cholesky of the
+// block and inverse of a triangular factor.
 //
-// Layout: the block's column c belongs to LPC consecutive lanes of one wave
-// (thread t: column t / LPC, lane-in-column sub = t % LPC); lane sub holds
-// rows sub, sub+LPC, sub+2*LPC, ... in registers (RPL = NL / LPC of them).
-// Values inside a column move between its lanes with wave shuffles, so the
-// triangular inverse needs no barrier at all and the factorization needs
-// one barrier per column (to publish the scaled column through LDS).
+// Layout: the block's column c belongs to 4 consecutive lanes of one wave
+// (thread t: column t / 4, lane-in-column sub = t % 4); lane sub holds rows
+// sub, sub+4, ..., sub+60 in registers. Loops over the 64 columns are fully
+// unrolled so every register index is a compile-time constant; values move
+// inside a column's lanes with wave shuffles. The triangular inverse needs no
+// barrier at all; the factorization needs one barrier per column, to publish
+// the scaled column through LDS.
 #pragma once
 
 #include <cmath>
@@ -16,45 +17,34 @@
 namespace mops {
 namespace leaf {
 
-template <int RPL>
-__device__ __forceinline__ double pick(const double (&v)[RPL], int q) {
-  double r = 0.0;
-#pragma unroll
-  for (int i = 0; i < RPL; ++i)
-    if (i == q) r = v[i];
-  return r;
-}
+constexpr int NL = 64;   // block order
+constexpr int LPC = 4;   // lanes per column
+constexpr int RPL = 16;  // rows per lane
+constexpr int LDL = NL + 1;
 
-template <int RPL>
-__device__ __forceinline__ void put(double (&v)[RPL], int q, double x) {
-#pragma unroll
-  for (int i = 0; i < RPL; ++i)
-    if (i == q) v[i] = x;
-}
-
-// Column c (rows sub + LPC*q) of inv(L), L lower triangular in LDS (row
-// stride LDL). unit: the diagonal is taken as 1 and not read. n: order.
-// Forward substitution L x = e_c with the solved component broadcast inside
-// the column's lanes (divisions as in the reference's substitution).
-template <int NL, int LPC, int LDL>
-__device__ __forceinline__ void inv_lower_column(const double* Ls, int n, int c, int sub,
-                                                 bool unit, double (&x)[NL / LPC]) {
-  constexpr int RPL = NL / LPC;
+// Column c (rows sub + 4q) of inv(L), L lower triangular in LDS (row pitch
+// LDL), order n <= 64. dinv[k] = 1 / L[k][k] (ignored when unit).
+__device__ __forceinline__ void inv_lower_column(const double* Ls, const double* dinv, int n,
+                                                 int c, int sub, bool unit, double (&x)[RPL]) {
   const int lane = threadIdx.x & 63;
   const int base = lane & ~(LPC - 1);
 #pragma unroll
   for (int q = 0; q < RPL; ++q) x[q] = (sub + LPC * q == c) ? 1.0 : 0.0;
-  for (int k = 0; k < n; ++k) {
-    double xk = __shfl(pick<RPL>(x, k / LPC), base | (k % LPC), 64);
-    if (k < c)
-      xk = 0.0;  // above the diagonal of column c
-    else if (!unit)
-      xk = xk / Ls[k * LDL + k];
-    if (sub == k % LPC) put<RPL>(x, k / LPC, xk);
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
-      const int r = sub + LPC * q;
-      if (r > k && r < n) x[q] -= Ls[r * LDL + k] * xk;
+  for (int K = 0; K < RPL; ++K) {
+#pragma unroll
+    for (int sp = 0; sp < LPC; ++sp) {
+      const int k = LPC * K + sp;
+      if (k < n) {
+        double xk = __shfl(x[K], base | sp, 64);
+        xk = k < c ? 0.0 : (unit ? xk : xk * dinv[k]);
+        if (sub == sp) x[K] = xk;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+          const int r = sub + LPC * q;
+          if (r > k && r < n) x[q] -= Ls[r * LDL + k] * xk;
+        }
+      }
     }
   }
 }

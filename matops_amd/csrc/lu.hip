@@ -17,7 +17,7 @@
 // the panel is a contiguous run. A copy-transpose brings A in and out.
 //
 // Panel (M x w, rows k..m-1): recursive halving down to LEAF_W-wide leaves.
-// A leaf is one cooperative kernel: G workgroups each hold up to 256 rows of
+// A leaf is one cooperative kernel: G workgroups each hold up to 512 rows of
 // the leaf in LDS; per column every workgroup finds its best candidate
 // (wave shuffles), publishes it with the candidate row, and after one grid
 // barrier all of them agree on the pivot, swap the two rows (in LDS for the
@@ -35,9 +35,9 @@
 namespace mops {
 namespace {
 
-constexpr int LEAF_W = 32;     // leaf width (columns)
-constexpr int LEAF_R = 256;    // rows per workgroup in a leaf
-constexpr int LEAF_T = 256;    // threads per leaf workgroup
+constexpr int LEAF_W = 32;   // leaf width (columns)
+constexpr int LEAF_R = 512;  // rows per workgroup in a leaf (= threads)
+constexpr int LEAF_GMAX = 64;
 constexpr unsigned SPIN_LIMIT = 1u << 26;
 
 // ---- copy-transpose of a column-major tile (64x64 through LDS) ----
@@ -58,11 +58,7 @@ __global__ __launch_bounds__(256) void k_transpose(int rows, int cols, const S* 
   }
 }
 
-// pivot candidate: value key (NaN mapped), row; larger key wins, ties -> lower row
-struct Cand {
-  double v;
-  int r;
-};
+// pivot candidates: larger key wins, ties -> lower row
 __device__ __forceinline__ bool better(double v1, int r1, double v2, int r2) {
   return v1 > v2 || (v1 == v2 && r1 < r2);
 }
@@ -75,21 +71,21 @@ __device__ __forceinline__ double ld_agent(const double* p) {
 }
 
 struct LeafArgs {
-  double* T;       // A^T, column-major, T[c + r*ldT] = A[r][c]
+  double* T;       // A^T, column-major: T[c + r*ldT] = A[r][c]
   long ldT;
-  int n;           // columns of A (= rows of T)
   int m;           // rows of A
-  int c0;          // first column of the leaf == first row of the panel
+  int c0;          // first column of the leaf == first row it pivots
   int w;           // leaf width (<= LEAF_W)
+  int pc0, pc1;    // columns of the enclosing panel (rows swapped in them too)
   int* piv;        // global pivots, piv[c0 + j]
-  int* perm;       // row permutation (perm[i] = original row now at i), or null
-  double* xb;      // exchange: [2][G][LEAF_W + 3]
+  int* perm;       // perm[i] = original row now at position i
+  double* xb;      // exchange slots: [2][G][LEAF_W + 3]
   double* rowk;    // exchange: [2][LEAF_W + 1]
   unsigned* ctr;   // grid barrier counter (zeroed before launch)
   int* err;        // set to 1 if a barrier wait timed out
 };
 
-// Grid barrier among the gridDim.x workgroups of a cooperative launch.
+// Grid barrier among the workgroups of a cooperative launch.
 __device__ __forceinline__ void grid_sync(unsigned* ctr, unsigned target, int* err) {
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -106,43 +102,49 @@ __device__ __forceinline__ void grid_sync(unsigned* ctr, unsigned target, int* e
   __syncthreads();
 }
 
-__global__ __launch_bounds__(LEAF_T) void k_getf2_leaf(LeafArgs a) {
-  constexpr int W = LEAF_W, LDW = LEAF_W + 1;
+// Pivoted LU of the leaf columns [c0, c0+w) over rows c0..m-1. Each of the G
+// workgroups keeps up to 512 rows of the leaf in LDS, one row per thread.
+// Per column: local best candidate (wave shuffles), publication of the
+// candidate and its row, one grid barrier, then every workgroup reads all
+// candidates at once, agrees on the pivot, swaps the two rows (in LDS for
+// the leaf, in global memory for the other columns of the panel) and updates
+// its rows. Rows of the matrix outside the panel are swapped later, all at
+// once, from the permutation.
+__global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
+  constexpr int W = LEAF_W, LDW = LEAF_W + 1, SW = LEAF_W + 3;
   __shared__ double tile[LEAF_R * LDW];
+  __shared__ double xs[LEAF_GMAX * SW];
   __shared__ double prow[W + 1];
   __shared__ double okrow[W + 1];
-  __shared__ int shp[2];
-  __shared__ double wv[LEAF_T / 64];
-  __shared__ int wr[LEAF_T / 64];
+  __shared__ double wv[LEAF_R / 64];
+  __shared__ int wr[LEAF_R / 64];
   __shared__ int lperm[LEAF_R];
+  __shared__ int shp;
 
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int r0 = a.c0;  // panel top row
-  const int M = a.m - r0;
+  const int r0 = a.c0;
   const int rg0 = r0 + g * LEAF_R;
   const int nr = max(0, min(LEAF_R, a.m - rg0));
   const int w = a.w;
   double* T = a.T;
   const long ldT = a.ldT;
 
-  // load my rows of the leaf: tile[r][c] = A[rg0 + r][c0 + c]
-  for (int idx = t; idx < LEAF_R * W; idx += LEAF_T) {
-    const int r = idx / W, c = idx % W;
-    double v = 0.0;
-    if (r < nr && c < w) v = T[(a.c0 + c) + (long)(rg0 + r) * ldT];
-    tile[r * LDW + c] = v;
+  if (t < nr) {
+    const double* src = T + a.c0 + (long)(rg0 + t) * ldT;
+    for (int c = 0; c < W; ++c) tile[t * LDW + c] = c < w ? src[c] : 0.0;
+    lperm[t] = a.perm[rg0 + t];
   }
-  if (t < LEAF_R) lperm[t] = (a.perm && t < nr) ? a.perm[rg0 + t] : rg0 + t;
   __syncthreads();
-  (void)M;
 
-  const int S = (a.n + G - 1) / G;  // slice of other columns swapped by this WG
+  // columns of the panel outside the leaf, sliced across the workgroups
+  const int ncol = (a.pc1 - a.pc0) - w;
+  const int slice = (ncol + G - 1) / G;
   for (int j = 0; j < w; ++j) {
-    const int k = r0 + j;  // diagonal row
-    double* xbj = a.xb + (long)(j & 1) * G * (W + 3);
+    const int k = r0 + j;
+    double* xbj = a.xb + (long)(j & 1) * G * SW;
     double* rkj = a.rowk + (j & 1) * (W + 1);
-    // ---- 1. local candidate among my rows >= k
+    // 1. local candidate among my rows >= k
     double v = -2.0;
     int r = INT_MAX;
     if (t < nr) {
@@ -167,21 +169,17 @@ __global__ __launch_bounds__(LEAF_T) void k_getf2_leaf(LeafArgs a) {
     }
     __syncthreads();
     if (t == 0) {
-      double bv = wv[0];
-      int br = wr[0];
-      for (int q = 1; q < LEAF_T / 64; ++q)
-        if (better(wv[q], wr[q], bv, br)) {
-          bv = wv[q];
-          br = wr[q];
+      for (int q = 1; q < LEAF_R / 64; ++q)
+        if (better(wv[q], wr[q], wv[0], wr[0])) {
+          wv[0] = wv[q];
+          wr[0] = wr[q];
         }
-      wv[0] = bv;
-      wr[0] = br;
     }
     __syncthreads();
-    // ---- 2. publish candidate (value, row, its leaf row, its perm entry)
+    // 2. publish: value, row, the row's leaf entries and its perm entry
     {
       const int br = wr[0];
-      double* slot = xbj + (long)g * (W + 3);
+      double* slot = xbj + (long)g * SW;
       if (t == 0) {
         st_agent(&slot[0], wv[0]);
         st_agent(&slot[1], (double)br);
@@ -193,19 +191,21 @@ __global__ __launch_bounds__(LEAF_T) void k_getf2_leaf(LeafArgs a) {
       }
       if (k >= rg0 && k < rg0 + nr) {
         const int lr = k - rg0;
-        if (t < W) st_agent(&rkj[t], tile[lr * LDW + t]);
-        if (t == W) st_agent(&rkj[W], (double)lperm[lr]);
+        if (t >= 64 && t < 64 + W) st_agent(&rkj[t - 64], tile[lr * LDW + t - 64]);
+        if (t == 64 + W) st_agent(&rkj[W], (double)lperm[lr]);
       }
     }
     grid_sync(a.ctr, (unsigned)(j + 1) * (unsigned)G, a.err);
-    // ---- 3. global pivot
-    {
+    // 3. read every candidate slot and the old row k in one round trip
+    for (int idx = t; idx < G * SW; idx += LEAF_R) xs[idx] = ld_agent(&xbj[idx]);
+    if (t >= LEAF_R - (W + 1)) okrow[t - (LEAF_R - (W + 1))] = ld_agent(&rkj[t - (LEAF_R - (W + 1))]);
+    __syncthreads();
+    if (wave == 0) {
       double bv = -3.0;
       int br = INT_MAX;
-      for (int q = t; q < G; q += LEAF_T) {
-        const double* slot = xbj + (long)q * (W + 3);
-        const double sv = ld_agent(&slot[0]);
-        const int sr = (int)ld_agent(&slot[1]);
+      for (int q = lane; q < G; q += 64) {
+        const double sv = xs[q * SW];
+        const int sr = (int)xs[q * SW + 1];
         if (better(sv, sr, bv, br)) {
           bv = sv;
           br = sr;
@@ -220,75 +220,62 @@ __global__ __launch_bounds__(LEAF_T) void k_getf2_leaf(LeafArgs a) {
         }
       }
       if (lane == 0) {
-        wv[wave] = bv;
-        wr[wave] = br;
-      }
-      __syncthreads();
-      if (t == 0) {
-        for (int q = 1; q < LEAF_T / 64; ++q)
-          if (better(wv[q], wr[q], wv[0], wr[0])) {
-            wv[0] = wv[q];
-            wr[0] = wr[q];
-          }
-        int p = wr[0];
-        if (p == INT_MAX) p = k;  // no candidate at all (cannot happen for k < m)
-        shp[0] = p;
+        const int p = br == INT_MAX ? k : br;
+        shp = p;
         if (g == 0) a.piv[k] = p;
-      }
-      __syncthreads();
-      const int p = shp[0];
-      const int gp = (p - r0) / LEAF_R;
-      const double* ws = xbj + (long)gp * (W + 3);
-      if (t <= W) {
-        prow[t] = ld_agent(&ws[2 + t]);  // pivot row (+ its perm entry)
-        okrow[t] = ld_agent(&rkj[t]);    // old row k (+ its perm entry)
       }
     }
     __syncthreads();
-    const int p = shp[0];
-    // ---- 4. swap rows k and p
+    const int p = shp;
+    const int gp = (p - r0) / LEAF_R;
+    if (t <= W) prow[t] = xs[gp * SW + 2 + t];
+    __syncthreads();
+    // 4. interchange rows k and p
     if (p != k) {
-      if (p >= rg0 && p < rg0 + nr) {
+      if (p >= rg0 && p < rg0 + nr && t <= W) {
         const int lr = p - rg0;
-        if (t < W) tile[lr * LDW + t] = okrow[t];
-        if (t == W) lperm[lr] = (int)okrow[W];
+        if (t < W)
+          tile[lr * LDW + t] = okrow[t];
+        else
+          lperm[lr] = (int)okrow[W];
       }
-      if (k >= rg0 && k < rg0 + nr) {
+      if (k >= rg0 && k < rg0 + nr && t <= W) {
         const int lr = k - rg0;
-        if (t < W) tile[lr * LDW + t] = prow[t];
-        if (t == W) lperm[lr] = (int)prow[W];
+        if (t < W)
+          tile[lr * LDW + t] = prow[t];
+        else
+          lperm[lr] = (int)prow[W];
       }
-      // the same interchange in every other column of A (T rows), my slice
-      const int cs = g * S, ce = min(a.n, cs + S);
+      // the other columns of the panel, my slice
       double* Tk = T + (long)k * ldT;
       double* Tp = T + (long)p * ldT;
-      for (int c = cs + t; c < ce; c += LEAF_T) {
-        if (c >= a.c0 && c < a.c0 + w) continue;
+      for (int q = g * slice + t; q < min(ncol, (g + 1) * slice); q += LEAF_R) {
+        int c = a.pc0 + q;
+        if (c >= a.c0) c += w;
         const double x = Tk[c];
         Tk[c] = Tp[c];
         Tp[c] = x;
       }
     }
     __syncthreads();
-    // ---- 5. column of L and rank-1 update of my rows below k
-    const double pv = prow[j];
+    // 5. column of L and rank-1 update of my rows below k
     if (t < nr) {
       const int gr = rg0 + t;
       if (gr > k) {
         double* row = &tile[t * LDW];
-        const double l = row[j] / pv;
+        const double pinv = 1.0 / prow[j];
+        const double l = row[j] * pinv;
         row[j] = l;
         for (int c = j + 1; c < w; ++c) row[c] -= l * prow[c];
       }
     }
     __syncthreads();
   }
-  // write my rows back
-  for (int idx = t; idx < LEAF_R * W; idx += LEAF_T) {
-    const int r = idx / W, c = idx % W;
-    if (r < nr && c < w) T[(a.c0 + c) + (long)(rg0 + r) * ldT] = tile[r * LDW + c];
+  if (t < nr) {
+    double* dst = T + a.c0 + (long)(rg0 + t) * ldT;
+    for (int c = 0; c < w; ++c) dst[c] = tile[t * LDW + c];
+    a.perm[rg0 + t] = lperm[t];
   }
-  if (a.perm && t < nr) a.perm[rg0 + t] = lperm[t];
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -302,6 +289,50 @@ __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 __global__ void k_piv_to_i64(const int* p, long long* q, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     q[i] = p[i];
+}
+
+// Net row movement of one panel: inv0[original row] = position before the
+// panel (rows k..m-1), then pairs (dst, src) for every position whose row
+// changed; pairs[0] holds the count.
+__global__ void k_inv_perm(const int* perm, int k, int m, int* inv0) {
+  for (int r = k + blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x)
+    inv0[perm[r]] = r;
+}
+__global__ void k_zero1(int* p) { p[0] = 0; }
+__global__ void k_perm_pairs(const int* perm, const int* inv0, int k, int m, int* pairs,
+                             int maxpairs) {
+  for (int r = k + blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+    const int src = inv0[perm[r]];
+    if (src != r) {
+      const int q = atomicAdd(&pairs[0], 1);
+      if (q < maxpairs) {
+        pairs[1 + 2 * q] = r;
+        pairs[2 + 2 * q] = src;
+      }
+    }
+  }
+}
+
+// Moves the rows listed in pairs (row dst of A receives old row src) in the
+// columns [c_lo, c_hi) of A, i.e. T-rows: every workgroup takes 16 of them,
+// reads all source values into LDS, then writes the destinations.
+constexpr int AP_ROWS = 16;
+__global__ __launch_bounds__(256) void k_apply_pairs(double* T, long ldT, const int* pairs,
+                                                     int maxpairs, int c_lo, int c_hi) {
+  extern __shared__ double vals[];  // [maxpairs][AP_ROWS]
+  const int np = min(pairs[0], maxpairs);
+  const int cbase = c_lo + blockIdx.x * AP_ROWS;
+  const int nc = min(AP_ROWS, c_hi - cbase);
+  if (nc <= 0 || np == 0) return;
+  for (int idx = threadIdx.x; idx < np * AP_ROWS; idx += 256) {
+    const int q = idx / AP_ROWS, c = idx % AP_ROWS;
+    if (c < nc) vals[idx] = T[cbase + c + (long)pairs[2 + 2 * q] * ldT];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < np * AP_ROWS; idx += 256) {
+    const int q = idx / AP_ROWS, c = idx % AP_ROWS;
+    if (c < nc) T[cbase + c + (long)pairs[1 + 2 * q] * ldT] = vals[idx];
+  }
 }
 
 // B_out[i][c] = B_in[perm[i]][c]  (gather rows of a column-major n x nrhs)
@@ -358,30 +389,39 @@ inline int grid1(long total) {
 }
 
 struct LuCtx {
-  hipStream_t s;
   double* T;
   long ldT;
   int m, n;
   int* piv;
   int* perm;
+  int* inv0;
+  int* pairs[2];
+  int maxpairs;
+  double* linv[2];  // inverse of the unit upper T11 of a panel (nb x nb)
+  double* tmp[2];   // U12 staging for the two streams
   double* xb;
   double* rowk;
   unsigned* ctr;
   int* err;
 };
 
-int leaf(LuCtx& c, int c0, int w) {
+int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   const int M = c.m - c0;
   const int G = ceil_div(M, LEAF_R);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, c.s, c.ctr);
+  if (G > LEAF_GMAX) {
+    set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
+    return ST_BADARG;
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
   MOPS_CHECK_HIP(hipGetLastError());
   LeafArgs a;
   a.T = c.T;
   a.ldT = c.ldT;
-  a.n = c.n;
   a.m = c.m;
   a.c0 = c0;
   a.w = w;
+  a.pc0 = pc0;
+  a.pc1 = pc1;
   a.piv = c.piv;
   a.perm = c.perm;
   a.xb = c.xb;
@@ -389,29 +429,80 @@ int leaf(LuCtx& c, int c0, int w) {
   a.ctr = c.ctr;
   a.err = c.err;
   void* args[] = {&a};
-  MOPS_CHECK_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_leaf, dim3(G), dim3(LEAF_T),
-                                            args, 0, c.s));
+  MOPS_CHECK_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_leaf, dim3(G), dim3(LEAF_R),
+                                            args, 0, s));
   return ST_OK;
 }
 
-// recursive factorization of the panel of columns [c0, c0+w), rows c0..m-1
-int panel(LuCtx& c, int c0, int w) {
-  if (w <= LEAF_W) return leaf(c, c0, w);
+// recursive factorization of the panel columns [c0, c0+w) inside the panel
+// [pc0, pc1), rows c0..m-1
+int panel_rec(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
+  if (w <= LEAF_W) return leaf(s, c, c0, w, pc0, pc1);
   int h = ((w / 2 + LEAF_W - 1) / LEAF_W) * LEAF_W;
   if (h >= w) h = w - LEAF_W;
-  MOPS_TRY(panel(c, c0, h));
+  MOPS_TRY(panel_rec(s, c, c0, h, pc0, pc1));
   const long ld = c.ldT;
   double* T = c.T;
-  // U block of the right half: X = A[c0:c0+h, c0+h:c0+w] = L11^-1 X, i.e. in
-  // T: X^T <- X^T * (L11^T)^-1 with L11^T unit upper in T
-  MOPS_TRY(trsm_d(c.s, F_RIGHT | F_UPPER | F_UNIT, h, w - h, 1.0, T + c0 + (long)c0 * ld, ld,
+  // U block of the right half: A[c0:c0+h, c0+h:c0+w] = L11^-1 (.), i.e. in
+  // T: X <- X * (L11^T)^-1, L11^T unit upper in T
+  MOPS_TRY(trsm_d(s, F_RIGHT | F_UPPER | F_UNIT, h, w - h, 1.0, T + c0 + (long)c0 * ld, ld,
                   T + (c0 + h) + (long)c0 * ld, ld));
   const int below = c.m - c0 - h;
   if (below > 0)
-    MOPS_TRY(gemm_d(c.s, false, false, w - h, below, h, -1.0, T + (c0 + h) + (long)c0 * ld, ld,
+    MOPS_TRY(gemm_d(s, false, false, w - h, below, h, -1.0, T + (c0 + h) + (long)c0 * ld, ld,
                     T + c0 + (long)(c0 + h) * ld, ld, 1.0, T + (c0 + h) + (long)(c0 + h) * ld,
                     ld));
-  if (c0 + h < c.m) MOPS_TRY(panel(c, c0 + h, w - h));
+  if (c0 + h < c.m) MOPS_TRY(panel_rec(s, c, c0 + h, w - h, pc0, pc1));
+  return ST_OK;
+}
+
+// Panel [k, k+kb): factor it, derive its row movement (pairs[par]) and the
+// inverse of its unit upper T11 (linv[par]).
+int panel(hipStream_t s, LuCtx& c, int k, int kb, int par) {
+  const int rows = c.m - k;
+  hipLaunchKernelGGL(k_inv_perm, dim3(grid1(rows)), dim3(256), 0, s, c.perm, k, c.m, c.inv0);
+  MOPS_CHECK_HIP(hipGetLastError());
+  trace_mark_begin(s, TR_PANEL);
+  MOPS_TRY(panel_rec(s, c, k, kb, k, k + kb));
+  trace_mark_end(s, TR_PANEL, 0.0);
+  hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, c.pairs[par]);
+  hipLaunchKernelGGL(k_perm_pairs, dim3(grid1(rows)), dim3(256), 0, s, c.perm, c.inv0, k, c.m,
+                     c.pairs[par], c.maxpairs);
+  MOPS_CHECK_HIP(hipGetLastError());
+  const size_t mark = ws_mark();
+  const int st = trtri_d(s, false, true, kb, c.T + k + (long)k * c.ldT, c.ldT, c.linv[par], kb);
+  ws_release(mark);
+  return st;
+}
+
+int apply_rows(hipStream_t s, LuCtx& c, int par, int c_lo, int c_hi) {
+  if (c_hi <= c_lo) return ST_OK;
+  const size_t shm = (size_t)c.maxpairs * AP_ROWS * sizeof(double);
+  hipLaunchKernelGGL(k_apply_pairs, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm, s, c.T,
+                     c.ldT, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+// Columns [j0, j1) of A after panel [k, k+kb): row interchanges, U12 = L11^-1
+// A12 (by the inverse), and the update A22 -= L21 U12 of rows k+kb..m-1.
+int update_cols(hipStream_t s, LuCtx& c, int k, int kb, int par, int j0, int j1, double* tmp,
+                bool gemm_trace) {
+  if (j1 <= j0) return ST_OK;
+  MOPS_TRY(apply_rows(s, c, par, j0, j1));
+  const long ld = c.ldT;
+  double* T = c.T;
+  const int nc = j1 - j0;
+  double* U = T + j0 + (long)k * ld;  // T rows j0..j1, T columns k..k+kb
+  MOPS_TRY(copy_tile_d(s, nc, kb, U, ld, tmp, nc));
+  MOPS_TRY(gemm_d(s, false, false, nc, kb, kb, 1.0, tmp, nc, c.linv[par], kb, 0.0, U, ld, GM_FULL,
+                  TRI_B_UPPER));
+  const int below = c.m - k - kb;
+  if (below <= 0) return ST_OK;
+  if (gemm_trace) trace_mark_begin(s, TR_GEMM_UPDATE);
+  MOPS_TRY(gemm_d(s, false, false, nc, below, kb, -1.0, U, ld, T + k + (long)(k + kb) * ld, ld,
+                  1.0, T + j0 + (long)(k + kb) * ld, ld));
+  if (gemm_trace) trace_mark_end(s, TR_GEMM_UPDATE, 2.0 * nc * (double)below * kb);
   return ST_OK;
 }
 
@@ -419,13 +510,17 @@ int panel(LuCtx& c, int c0, int w) {
 
 size_t getrf_ws_bytes(int m, int n, int nb) {
   if (m <= 0 || n <= 0) return 0;
-  if (nb <= 0 || nb > (m < n ? m : n)) nb = m < n ? m : n;
   const int mn = m < n ? m : n;
+  if (nb <= 0 || nb > 256) nb = 256;
+  if (nb > mn) nb = mn;
   const int G = ceil_div(m, LEAF_R);
-  size_t b = (size_t)m * n * sizeof(double);                     // T
-  b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;     // exchange
-  b += (size_t)(mn + m + 64) * sizeof(int);                      // piv, perm, ctr, err
-  b += trsm_ws_bytes(F_RIGHT, nb > LEAF_W ? nb : LEAF_W, n) + 8 * 256;
+  size_t b = (size_t)m * n * sizeof(double);                   // T
+  b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;   // exchange
+  b += (size_t)(mn + 3 * m + 64) * sizeof(int);                // piv, perm, inv0, ctr, err
+  b += (size_t)2 * (4 * nb + 2) * sizeof(int);                 // pairs
+  b += (size_t)2 * nb * nb * sizeof(double);                   // linv
+  b += (size_t)2 * n * nb * sizeof(double);                    // tmp
+  b += trtri_ws_bytes(nb) + trsm_ws_bytes(F_RIGHT, nb, nb) + 32 * 256;
   return b;
 }
 
@@ -433,23 +528,37 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
             int* perm_out, int* err_flag) {
   if (m <= 0 || n <= 0) return ST_OK;
   const int mn = m < n ? m : n;
-  // DecomposeLU: unblocked iff min(m,n) <= nb || nb == 0 (lu.go:267); both
-  // orders produce the same factorization, here one panel covers it all
-  if (nb <= 0 || nb >= mn) nb = mn;
+  // DecomposeLU: unblocked iff min(m,n) <= nb || nb == 0 (lu.go:267). Every
+  // blocking gives the same factorization (same pivots, to rounding), so the
+  // GPU schedule uses panels of at most 256 columns whatever nb is.
+  if (nb <= 0 || nb > 256) nb = 256;
+  if (nb > mn) nb = mn;
   const int G = ceil_div(m, LEAF_R);
+  if (G > LEAF_GMAX) {
+    set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
+    return ST_BADARG;
+  }
   LuCtx c;
-  c.s = s;
   c.m = m;
   c.n = n;
   c.ldT = n;
+  c.maxpairs = 2 * nb;
   c.T = (double*)ws_carve((size_t)m * n * sizeof(double));
   c.xb = (double*)ws_carve((size_t)2 * G * (LEAF_W + 3) * sizeof(double));
   c.rowk = (double*)ws_carve(2 * (LEAF_W + 1) * sizeof(double));
   c.piv = (int*)ws_carve((size_t)mn * sizeof(int));
   c.perm = (int*)ws_carve((size_t)m * sizeof(int));
+  c.inv0 = (int*)ws_carve((size_t)m * sizeof(int));
+  c.pairs[0] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
+  c.pairs[1] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
+  c.linv[0] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
+  c.linv[1] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
+  c.tmp[0] = (double*)ws_carve((size_t)n * nb * sizeof(double));
+  c.tmp[1] = (double*)ws_carve((size_t)n * nb * sizeof(double));
   c.ctr = (unsigned*)ws_carve(256);
   c.err = err_flag ? err_flag : (int*)ws_carve(256);
-  if (!c.T || !c.xb || !c.rowk || !c.piv || !c.perm || !c.ctr || !c.err) {
+  if (!c.T || !c.xb || !c.rowk || !c.piv || !c.perm || !c.inv0 || !c.pairs[0] || !c.pairs[1] ||
+      !c.linv[0] || !c.linv[1] || !c.tmp[0] || !c.tmp[1] || !c.ctr || !c.err) {
     set_error("getrf: workspace too small");
     return ST_NOMEM;
   }
@@ -459,27 +568,46 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   hipLaunchKernelGGL((k_transpose<double, double>), dim3(ceil_div(m, 64), ceil_div(n, 64)),
                      dim3(256), 0, s, m, n, A, lda, c.T, c.ldT);
   MOPS_CHECK_HIP(hipGetLastError());
-  for (int k = 0; k < mn; k += nb) {
+
+  // Look-ahead schedule. Stream s2 factors panel k+1 right after it has
+  // brought that panel's columns up to date with panel k ("next"), while
+  // the caller's stream s applies panel k to everything else ("rest"):
+  // the earlier columns (row interchanges only) and the columns right of
+  // panel k+1 (interchanges, U12, trailing GEMM).
+  hipStream_t s2 = aux_stream();
+  hipEvent_t e_fork = sync_event(0), e_pan = sync_event(1), e_rest = sync_event(2),
+             e_join = sync_event(3);
+  if (!s2 || !e_fork || !e_pan || !e_rest || !e_join) {
+    set_error("getrf: could not create the look-ahead stream");
+    return ST_HIP;
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
+  MOPS_TRY(panel(s2, c, 0, min(nb, mn), 0));
+  MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
+  bool have_rest = false;
+  int par = 0;
+  for (int k = 0; k < mn; k += nb, par ^= 1) {
     const int kb = min(nb, mn - k);
-    trace_mark_begin(s, TR_PANEL);
-    MOPS_TRY(panel(c, k, kb));
-    trace_mark_end(s, TR_PANEL, 0.0);
-    if (k + kb < n) {
-      double* T = c.T;
-      const long ld = c.ldT;
-      // U12: A[k:k+kb, k+kb:n] = L11^-1 A12
-      MOPS_TRY(trsm_d(s, F_RIGHT | F_UPPER | F_UNIT, kb, n - k - kb, 1.0, T + k + (long)k * ld,
-                      ld, T + (k + kb) + (long)k * ld, ld));
-      if (k + kb < m) {
-        // A22 -= L21 U12  <=>  T22 -= T12 * T21
-        trace_mark_begin(s, TR_GEMM_UPDATE);
-        MOPS_TRY(gemm_d(s, false, false, n - k - kb, m - k - kb, kb, -1.0,
-                        T + (k + kb) + (long)k * ld, ld, T + k + (long)(k + kb) * ld, ld, 1.0,
-                        T + (k + kb) + (long)(k + kb) * ld, ld));
-        trace_mark_end(s, TR_GEMM_UPDATE, 2.0 * (double)(n - k - kb) * (m - k - kb) * kb);
-      }
+    const int kb1 = min(nb, mn - k - kb);  // width of the next panel (0 at the end)
+    const int j_next = k + kb, j_rest = k + kb + kb1;
+    if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
+    MOPS_TRY(apply_rows(s, c, par, 0, k));
+    if (j_rest < n) {
+      MOPS_TRY(update_cols(s, c, k, kb, par, j_rest, n, c.tmp[0], true));
+      MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
+      have_rest = true;
+    }
+    if (kb1 > 0) {
+      MOPS_TRY(update_cols(s2, c, k, kb, par, j_next, j_next + kb1, c.tmp[1], false));
+      MOPS_TRY(panel(s2, c, j_next, kb1, par ^ 1));
+      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     }
   }
+  MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
+
   // A = T^T
   hipLaunchKernelGGL((k_transpose<double, double>), dim3(ceil_div(n, 64), ceil_div(m, 64)),
                      dim3(256), 0, s, n, m, c.T, c.ldT, A, lda);

@@ -18,42 +18,43 @@
 namespace mops {
 namespace {
 
-constexpr int LEAF = 64;
-constexpr int LEAF_LPC = 4;
+constexpr int LEAF = leaf::NL;
 
 // Inverts the 64x64 (or smaller, at the end) diagonal leaves of nblk
 // blocks of order bb. Block b of A starts at A + b*sA, its inverse at
 // X + b*sX. The leaf also clears its columns of the other triangle inside
 // the block, so that X ends up dense with explicit zeros. The triangle is
 // staged in LDS (an upper one transposed, inv(U) = inv(U^T)^T) and each
-// column of the inverse is solved by LEAF_LPC lanes (leaf.hpp).
-__global__ __launch_bounds__(LEAF* LEAF_LPC) void k_trtri_leaf(int bb, int lower, int unit,
-                                                               const double* A, long lda, long sA,
-                                                               double* X, long ldx, long sX) {
-  constexpr int LDL = LEAF + 1;
-  constexpr int RPL = LEAF / LEAF_LPC;
-  __shared__ double Ls[LEAF * LDL];
+// column of the inverse is solved by 4 lanes (leaf.hpp).
+__global__ __launch_bounds__(256) void k_trtri_leaf(int bb, int lower, int unit,
+                                                    const double* A, long lda, long sA,
+                                                    double* X, long ldx, long sX) {
+  using namespace leaf;
+  __shared__ double Ls[NL * LDL];
+  __shared__ double dinv[NL];
   const int o = blockIdx.x * LEAF;
   const int lb = min(LEAF, bb - o);
   const double* Ab = A + blockIdx.y * sA + o + (long)o * lda;
   double* Xblk = X + blockIdx.y * sX;
   double* Xd = Xblk + o + (long)o * ldx;
   const int t = threadIdx.x;
-  const int c = t / LEAF_LPC, sub = t % LEAF_LPC;
+  const int c = t / LPC, sub = t % LPC;
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
-    const int r = sub + LEAF_LPC * q;
+    const int r = sub + LPC * q;
     double v = 0.0;
     if (r < lb && c < lb && r >= c && !(unit && r == c))
       v = lower ? Ab[r + (long)c * lda] : Ab[c + (long)r * lda];
     Ls[r * LDL + c] = v;
   }
   __syncthreads();
+  if (t < NL) dinv[t] = (t < lb && !unit) ? 1.0 / Ls[t * LDL + t] : 1.0;
+  __syncthreads();
   double x[RPL];
-  leaf::inv_lower_column<LEAF, LEAF_LPC, LDL>(Ls, lb, c, sub, unit != 0, x);
+  inv_lower_column(Ls, dinv, lb, c, sub, unit != 0, x);
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
-    const int r = sub + LEAF_LPC * q;
+    const int r = sub + LPC * q;
     if (r < lb && c < lb) {
       if (lower)
         Xd[r + (long)c * ldx] = x[q];
@@ -64,10 +65,10 @@ __global__ __launch_bounds__(LEAF* LEAF_LPC) void k_trtri_leaf(int bb, int lower
   // explicit zeros in this leaf's columns on the other side of the diagonal
   if (lower) {
     for (int cc = 0; cc < lb; ++cc)
-      for (int i = t; i < o; i += LEAF * LEAF_LPC) Xblk[i + (long)(o + cc) * ldx] = 0.0;
+      for (int i = t; i < o; i += 256) Xblk[i + (long)(o + cc) * ldx] = 0.0;
   } else {
     for (int cc = 0; cc < lb; ++cc)
-      for (int i = o + lb + t; i < bb; i += LEAF * LEAF_LPC) Xblk[i + (long)(o + cc) * ldx] = 0.0;
+      for (int i = o + lb + t; i < bb; i += 256) Xblk[i + (long)(o + cc) * ldx] = 0.0;
   }
 }
 
@@ -187,7 +188,7 @@ int trtri_core(hipStream_t s, bool lower, bool unit, int bb, int nblk, const dou
     set_error("trtri: too many blocks");
     return ST_BADARG;
   }
-  hipLaunchKernelGGL(k_trtri_leaf, dim3(ceil_div(bb, LEAF), nblk), dim3(LEAF * LEAF_LPC), 0, s, bb,
+  hipLaunchKernelGGL(k_trtri_leaf, dim3(ceil_div(bb, LEAF), nblk), dim3(256), 0, s, bb,
                      lower ? 1 : 0, unit ? 1 : 0, A, lda, sA, X, ldx, sX);
   MOPS_CHECK_HIP(hipGetLastError());
   // Level h joins the inverses of [o, o+h) and [o+h, min(o+2h, bb)).
