@@ -30,78 +30,124 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 
 constexpr unsigned LEAF_SPIN_LIMIT = 1u << 26;
 
-// One 64-column leaf of the Cholesky panel in one launch (64-thread
-// workgroups, one wave each).
-//  * workgroup 0 factors the n x n diagonal block (n <= 64): lane i holds row
-//    i in registers; column j takes the square root of the diagonal (a NaN
-//    records its global index, chol.go:33-36), scales the column by the
-//    reciprocal (InvScale, chol.go:42 -- built with -ffast-math, the
-//    reference's loop of divisions is a reciprocal multiply) and applies the
-//    rank-1 update to the rest of each row (the SYR of chol.go:44), the
-//    column entries broadcast with readlane. All loops are unrolled so the
-//    registers are addressed statically; the critical chain per column is a
-//    square root, a reciprocal and one multiply-add. The factor is stored
-//    and released to the other workgroups through a flag.
-//  * workgroup b >= 1 solves 64 rows below the block (X L^T = B, the TRSM of
-//    chol.go:79 for this leaf): lane = row, right-looking substitution with
-//    the factor broadcast from LDS.
+__device__ __forceinline__ double ld_agent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int CL_T = 256;  // threads per workgroup of the leaf kernel
+constexpr int CL_LD = LEAF + 1;
+
+// One 64-column leaf of the Cholesky panel in one launch (256 threads per
+// workgroup, 4 waves).
+//  * Workgroup 0 factors the n x n diagonal block (n <= 64; rows/columns
+//    beyond n are padded with the identity). Lane l of every wave holds row
+//    l; wave w holds columns 16w..16w+15 of the block and of its inverse M in
+//    registers (static indices: the column loop is unrolled 16-fold inside a
+//    rolled loop over the 4 column groups, which keeps the code small).
+//    Column j (the reference's column step, chol.go:29-46): the owning wave
+//    takes the square root of the diagonal (a NaN records its global index,
+//    chol.go:33-36), scales the column by the reciprocal (InvScale,
+//    chol.go:42; built with -ffast-math, the reference's divisions are a
+//    reciprocal multiply) and publishes it through LDS; after one barrier
+//    every wave applies the rank-1 update (the SYR of chol.go:44) to its
+//    columns right of j, and the same elimination step to the identity
+//    (row j of M scaled by the reciprocal, the rows below it updated), so
+//    that M = L^-1 when the loop ends. L goes back to A, M to Minv, and a
+//    flag releases both.
+//  * Workgroup b >= 1 loads 64 rows below the block (the TRSM of chol.go:79
+//    for this leaf: X L^T = B, X = B M^T) into LDS, waits for the flag and
+//    forms X = B M^T on the matrix cores (v_mfma_f64_16x16x4, skipping the
+//    zero upper triangle of M).
 // The UPPER case runs the same on U^T (rows of U^T = columns of U).
-__global__ __launch_bounds__(64) void k_chol_leaf(int n, int mb, int upper, double* A, long lda,
-                                                  double* B, int goff, int* info,
-                                                  double* badval, int* flag, int* err) {
-  constexpr int NL = 64;
-  const int l = threadIdx.x;
+__global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int upper, double* A, long lda,
+                                                    double* B, double* Minv, int goff, int* info,
+                                                    double* badval, int* flag, int* err) {
+  __shared__ double s1[LEAF * CL_LD];
+  __shared__ double s2[LEAF * CL_LD];
+  const int t = threadIdx.x, l = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   if (blockIdx.x == 0) {
-    double a[NL];
+    double a[16], m[16];
 #pragma unroll
-    for (int c = 0; c < NL; ++c) {
-      double v = 0.0;
+    for (int i = 0; i < 16; ++i) {
+      const int c = 16 * w + i;
+      double v = c == l ? 1.0 : 0.0;
       if (l < n && c < n && c <= l) v = upper ? A[c + (long)l * lda] : A[l + (long)c * lda];
-      a[c] = v;
+      a[i] = v;
+      m[i] = c == l ? 1.0 : 0.0;
     }
+#pragma unroll 1
+    for (int jb = 0; jb < 4; ++jb) {
 #pragma unroll
-    for (int j = 0; j < NL; ++j) {
-      if (j < n) {
-        const double d = readlane_d(a[j], j);
-        const double r = sqrt(d);
-        if (l == 0 && isnan(r) && info) {
-          const int old = atomicMin(info, goff + j);
-          if ((old < 0 || old > goff + j) && badval) *badval = d;
+      for (int ji = 0; ji < 16; ++ji) {
+        const int j = 16 * jb + ji;
+        double* cb = s1 + (ji & 1) * 72;  // column j (rows > j), [64] = 1 / L[j][j]
+        if (w == jb) {
+          const double d = readlane_d(a[ji], j);
+          const double r = sqrt(d);
+          if (l == 0 && isnan(r) && info && j < n) {
+            const int old = atomicMin(info, goff + j);
+            if ((old < 0 || old > goff + j) && badval) *badval = d;
+          }
+          const double ri = 1.0 / r;
+          const double x = a[ji] * ri;
+          a[ji] = l > j ? x : (l == j ? r : a[ji]);
+          cb[l] = l > j ? x : 0.0;
+          if (l == 0) cb[64] = ri;
         }
-        const double ri = 1.0 / r;
-        a[j] = l > j ? a[j] * ri : (l == j ? r : a[j]);
+        __syncthreads();
+        const double xl = cb[l];
+        const double ri = cb[64];
+        if (w > jb) {
 #pragma unroll
-        for (int k = j + 1; k < NL; ++k) {
-          if (k < n) {
-            const double lk = readlane_d(a[j], k);
-            if (l >= k) a[k] -= a[j] * lk;
+          for (int i = 0; i < 16; ++i) a[i] -= xl * cb[16 * w + i];
+        } else if (w == jb) {
+#pragma unroll
+          for (int i = ji + 1; i < 16; ++i) a[i] -= xl * cb[16 * w + i];
+        }
+        if (w < jb) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const double mj = readlane_d(m[i], j) * ri;
+            m[i] = l == j ? mj : m[i] - xl * mj;
+          }
+        } else if (w == jb) {
+#pragma unroll
+          for (int i = 0; i <= ji; ++i) {
+            const double mj = readlane_d(m[i], j) * ri;
+            m[i] = l == j ? mj : m[i] - xl * mj;
           }
         }
       }
     }
 #pragma unroll
-    for (int c = 0; c < NL; ++c)
+    for (int i = 0; i < 16; ++i) {
+      const int c = 16 * w + i;
       if (l < n && c < n && c <= l) {
         if (upper)
-          A[c + (long)l * lda] = a[c];
+          A[c + (long)l * lda] = a[i];
         else
-          A[l + (long)c * lda] = a[c];
+          A[l + (long)c * lda] = a[i];
       }
-    if (l == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      Minv[l + c * LEAF] = m[i];
+    }
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  // ---- rows below: row index of this lane
-  __shared__ double Ls[NL][NL + 1];
-  __shared__ double dinv[NL];
-  const int row = (blockIdx.x - 1) * NL + l;
-  double x[NL];
-#pragma unroll
-  for (int c = 0; c < NL; ++c) {
+  // ---- 64 rows below the diagonal block
+  double* Bs = s1;  // Bs[r * CL_LD + k] = B[row0 + r][k]
+  double* Ms = s2;  // Ms[c * CL_LD + k] = M[c][k]
+  const int row0 = (blockIdx.x - 1) * LEAF;
+  for (int idx = t; idx < LEAF * LEAF; idx += CL_T) {
+    const int a0 = idx % LEAF, a1 = idx / LEAF;  // contiguous index first
+    const int r = upper ? a1 : a0, c = upper ? a0 : a1;
     double v = 0.0;
-    if (row < mb && c < n) v = upper ? B[c + (long)row * lda] : B[row + (long)c * lda];
-    x[c] = v;
+    if (row0 + r < mb && c < n)
+      v = upper ? B[c + (long)(row0 + r) * lda] : B[(row0 + r) + (long)c * lda];
+    Bs[r * CL_LD + c] = v;
   }
-  if (l == 0) {
+  if (t == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
       if (++spins > LEAF_SPIN_LIMIT) {
@@ -112,38 +158,42 @@ __global__ __launch_bounds__(64) void k_chol_leaf(int n, int mb, int upper, doub
     }
   }
   __syncthreads();
-  // the factor, row by row: Ls[i][k] = L[i][k] (k <= i)
-  for (int idx = l; idx < NL * NL; idx += NL) {
-    const int i = idx % NL, k = idx / NL;
-    double v = 0.0;
-    if (i < n && k <= i)
-      v = upper ? __hip_atomic_load(&A[k + (long)i * lda], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT)
-                : __hip_atomic_load(&A[i + (long)k * lda], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    Ls[i][k] = v;
-  }
+  for (int idx = t; idx < LEAF * LEAF; idx += CL_T)
+    Ms[(idx % LEAF) * CL_LD + idx / LEAF] = ld_agent(&Minv[idx]);
   __syncthreads();
-  dinv[l] = l < n ? 1.0 / Ls[l][l] : 0.0;
-  __syncthreads();
+  // wave w: X^T rows c = 16w..16w+15 (all 64 rows r of X): D[c][r] =
+  // sum_k M[c][k] B[r][k], k <= 16w+15 (M is lower triangular)
+  v4d acc[4];
 #pragma unroll
-  for (int k = 0; k < NL; ++k) {
-    if (k < n) {
-      x[k] *= dinv[k];
+  for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int li = l % 16, lk = l / 16;
+  const int ksteps = 4 * w + 4;
+  for (int s = 0; s < ksteps; ++s) {
+    const int k = 4 * s + lk;
+    const double av = Ms[(16 * w + li) * CL_LD + k];
 #pragma unroll
-      for (int c = k + 1; c < NL; ++c)
-        if (c < n) x[c] -= x[k] * Ls[c][k];
+    for (int q = 0; q < 4; ++q) {
+      const double bv = Bs[(16 * q + li) * CL_LD + k];
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
     }
   }
-  if (row < mb) {
+  __syncthreads();
+  // D[c][r] at lane l, register e: c = 16w + l/16 + 4e, r = 16q + l%16
 #pragma unroll
-    for (int c = 0; c < NL; ++c)
-      if (c < n) {
-        if (upper)
-          B[c + (long)row * lda] = x[c];
-        else
-          B[row + (long)c * lda] = x[c];
-      }
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bs[(16 * q + li) * CL_LD + 16 * w + lk + 4 * e] = acc[q][e];
+  __syncthreads();
+  for (int idx = t; idx < LEAF * LEAF; idx += CL_T) {
+    const int a0 = idx % LEAF, a1 = idx / LEAF;
+    const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+    if (row0 + r < mb && c < n) {
+      const double v = Bs[r * CL_LD + c];
+      if (upper)
+        B[c + (long)(row0 + r) * lda] = v;
+      else
+        B[(row0 + r) + (long)c * lda] = v;
+    }
   }
 }
 
@@ -170,9 +220,10 @@ __global__ void k_fix_info(int* info, const int* err) {
 // diagonal block and the rows below it in one launch). The UPPER case is
 // the transpose of each step.
 struct LeafFlags {
-  int* flags;  // one zeroed flag per leaf launch
+  int* flags;    // one zeroed flag per leaf launch
   int next;
   int* err;
+  double* minv;  // 64 x 64 inverse of the current leaf's diagonal block
 };
 
 int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
@@ -183,8 +234,8 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
     double* B = upper ? A + j0 + (long)(j0 + w) * lda : A + (j0 + w) + (long)j0 * lda;
     int* flag = fl.flags + fl.next;
     fl.next += 1;
-    hipLaunchKernelGGL(k_chol_leaf, dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(LEAF), 0, s, w,
-                       m, upper ? 1 : 0, D, lda, B, j0, info, badval, flag, fl.err);
+    hipLaunchKernelGGL(k_chol_leaf, dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(CL_T), 0, s, w,
+                       m, upper ? 1 : 0, D, lda, B, fl.minv, j0, info, badval, flag, fl.err);
     MOPS_CHECK_HIP(hipGetLastError());
     return ST_OK;
   }
@@ -209,7 +260,8 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
 
 size_t potrf_ws_bytes(int N, int nb) {
   if (nb <= 0 || nb > N) nb = N;
-  return (size_t)(ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 8) * sizeof(int) + 1024;
+  return (size_t)(ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 8) * sizeof(int) +
+         (size_t)LEAF * LEAF * sizeof(double) + 2048;
 }
 
 namespace {
@@ -258,8 +310,9 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   const int nleaf = ceil_div(N, LEAF) + 2 * ceil_div(N, nb) + 4;
   LeafFlags fl;
   fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
+  fl.minv = (double*)ws_carve((size_t)LEAF * LEAF * sizeof(double));
   fl.next = 0;
-  if (!fl.flags) {
+  if (!fl.flags || !fl.minv) {
     set_error("potrf: workspace too small");
     return ST_NOMEM;
   }
