@@ -13,6 +13,7 @@
 // and solve the rows below with one GEMM; the halves are joined by a
 // trapezoidal GEMM update. The trailing update of step k runs on the
 // caller's stream while the next panel is factored on a second stream.
+#include <climits>
 #include <cmath>
 
 #include "mops_solvers.hpp"
@@ -37,152 +38,262 @@ __device__ __forceinline__ double ld_agent(const double* p) {
 constexpr int CL_T = 256;  // threads per workgroup of the leaf kernel
 constexpr int CL_LD = LEAF + 1;
 
+// 1/sqrt(d) for a finite d > 0: the hardware estimate (v_rsq_f64) refined
+// by two Newton-Raphson steps y <- y + y (1 - d y^2) / 2, each of which
+// roughly doubles the number of correct bits.
+__device__ __forceinline__ double rsqrt_newton(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  double e = fma(-d * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-d * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // One 64-column leaf of the Cholesky panel in one launch (256 threads per
 // workgroup, 4 waves).
-//  * Workgroup 0 factors the n x n diagonal block (n <= 64; rows/columns
-//    beyond n are padded with the identity). Lane l of every wave holds row
-//    l; wave w holds columns 16w..16w+15 of the block and of its inverse M in
-//    registers (static indices: the column loop is unrolled 16-fold inside a
-//    rolled loop over the 4 column groups, which keeps the code small).
-//    Column j (the reference's column step, chol.go:29-46): the owning wave
-//    takes the square root of the diagonal (a NaN records its global index,
-//    chol.go:33-36), scales the column by the reciprocal (InvScale,
-//    chol.go:42; built with -ffast-math, the reference's divisions are a
-//    reciprocal multiply) and publishes it through LDS; after one barrier
-//    every wave applies the rank-1 update (the SYR of chol.go:44) to its
-//    columns right of j, and the same elimination step to the identity
-//    (row j of M scaled by the reciprocal, the rows below it updated), so
-//    that M = L^-1 when the loop ends. L goes back to A, M to Minv, and a
-//    flag releases both.
-//  * Workgroup b >= 1 loads 64 rows below the block (the TRSM of chol.go:79
-//    for this leaf: X L^T = B, X = B M^T) into LDS, waits for the flag and
-//    forms X = B M^T on the matrix cores (v_mfma_f64_16x16x4, skipping the
-//    zero upper triangle of M).
+//
+// Workgroup 0 factors the n x n diagonal block (n <= 64; rows and columns
+// beyond n are padded with the identity). Wave w owns columns 16w..16w+15,
+// kept as four 16 x 16 MFMA accumulator tiles; the block is factored in four
+// 16-column steps:
+//  * wave jb moves its tiles to row layout (lane l = row l, 16 registers =
+//    columns; through LDS) and factors its 16 columns alone. Column j is the
+//    reference's column step (chol.go:29-46): L[j][j] = sqrt(d), the column
+//    below scaled by 1/L[j][j] (InvScale, chol.go:42; built with
+//    -ffast-math, the reference's divisions are a multiply by the
+//    reciprocal, here 1/sqrt(d) directly), then the rank-1 update of the
+//    wave's remaining columns (the SYR of chol.go:44) with the column
+//    broadcast from LDS. The next diagonal element is formed from two
+//    readlanes ahead of the full update, which keeps the serial chain per
+//    column short. The finished columns go to LDS (Lt) and to A;
+//  * after one barrier the waves right of it apply the rank-16 update of
+//    their tiles on the matrix cores (the same SYR steps for those 16
+//    columns, v_mfma_f64_16x16x4).
+// A NaN diagonal (chol.go:33-36) is reported once, for the first column.
+// Wave 0 then inverts the four 16 x 16 diagonal blocks (lane = block and
+// column of the inverse, forward substitution) into Minv, and a flag
+// releases L and the inverses.
+//
+// Workgroups b >= 1 load 64 rows below the block into LDS (B), wait for the
+// flag, and solve X L^T = B (the TRSM of chol.go:79 for this leaf) on the
+// matrix cores, 16 rows per wave, 16 columns at a time:
+//   X_b^T = inv(L_bb) (B_b^T - sum_{a<b} L_ba X_a^T)
+// The MFMA result layout of a solved block X_a^T is exactly the B-operand
+// layout the later blocks need, so solved blocks stay in registers.
 // The UPPER case runs the same on U^T (rows of U^T = columns of U).
 __global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int upper, double* A, long lda,
                                                     double* B, double* Minv, int goff, int* info,
                                                     double* badval, int* flag, int* err) {
   __shared__ double s1[LEAF * CL_LD];
   __shared__ double s2[LEAF * CL_LD];
+  __shared__ double dv[LEAF];  // WG0: diagonal before the square root
   const int t = threadIdx.x, l = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int li = l % 16, lq = l / 16;
   if (blockIdx.x == 0) {
-    double a[16], m[16];
+    double* Lt = s1;         // Lt[c * 64 + r] = L[r][c], filled block by block
+    double* Cs = s2;         // row-layout staging of the owner's tiles, pitch 17
+    double* rv = s2 + 1152;  // rv[j] = 1 / L[j][j]
+    // tile q of wave w: register e of lane l = A[16q + lq + 4e][16w + li]
+    v4d acc[4];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int c = 16 * w + i;
-      double v = c == l ? 1.0 : 0.0;
-      if (l < n && c < n && c <= l) v = upper ? A[c + (long)l * lda] : A[l + (long)c * lda];
-      a[i] = v;
-      m[i] = c == l ? 1.0 : 0.0;
-    }
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = 16 * q + lq + 4 * e, c = 16 * w + li;
+        double v = r == c ? 1.0 : 0.0;
+        if (r < n && c < n && c <= r) v = upper ? A[c + (long)r * lda] : A[r + (long)c * lda];
+        acc[q][e] = v;
+      }
 #pragma unroll 1
     for (int jb = 0; jb < 4; ++jb) {
+      if (w == jb) {
+        // tiles -> row layout a[i] = (row l, column 16jb + i)
 #pragma unroll
-      for (int ji = 0; ji < 16; ++ji) {
-        const int j = 16 * jb + ji;
-        double* cb = s1 + (ji & 1) * 72;  // column j (rows > j), [64] = 1 / L[j][j]
-        if (w == jb) {
-          const double d = readlane_d(a[ji], j);
-          const double r = sqrt(d);
-          if (l == 0 && isnan(r) && info && j < n) {
-            const int old = atomicMin(info, goff + j);
-            if ((old < 0 || old > goff + j) && badval) *badval = d;
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) Cs[(16 * q + lq + 4 * e) * 17 + li] = acc[q][e];
+        double a[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) a[i] = Cs[l * 17 + i];
+        double dn = readlane_d(a[0], 16 * jb);
+#pragma unroll
+        for (int ji = 0; ji < 16; ++ji) {
+          const int j = 16 * jb + ji;
+          const double d = dn;
+          double ri, r;
+          if (d > 0.0 && d < INFINITY) {
+            ri = rsqrt_newton(d);
+            r = d * ri;
+          } else {
+            r = sqrt(d);
+            ri = 1.0 / r;
           }
-          const double ri = 1.0 / r;
-          const double x = a[ji] * ri;
-          a[ji] = l > j ? x : (l == j ? r : a[ji]);
-          cb[l] = l > j ? x : 0.0;
-          if (l == 0) cb[64] = ri;
-        }
-        __syncthreads();
-        const double xl = cb[l];
-        const double ri = cb[64];
-        if (w > jb) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) a[i] -= xl * cb[16 * w + i];
-        } else if (w == jb) {
-#pragma unroll
-          for (int i = ji + 1; i < 16; ++i) a[i] -= xl * cb[16 * w + i];
-        }
-        if (w < jb) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const double mj = readlane_d(m[i], j) * ri;
-            m[i] = l == j ? mj : m[i] - xl * mj;
+          const double x = l > j ? a[ji] * ri : 0.0;
+          if (ji < 15) {
+            const double xn = readlane_d(x, j + 1);
+            dn = fma(-xn, xn, readlane_d(a[ji + 1], j + 1));
           }
-        } else if (w == jb) {
+          a[ji] = l == j ? r : x;
+          Lt[j * 64 + l] = a[ji];
+          dv[j] = d;
+          rv[j] = ri;
 #pragma unroll
-          for (int i = 0; i <= ji; ++i) {
-            const double mj = readlane_d(m[i], j) * ri;
-            m[i] = l == j ? mj : m[i] - xl * mj;
+          for (int i = ji + 1; i < 16; ++i) a[i] = fma(-x, Lt[j * 64 + 16 * jb + i], a[i]);
+          // keep this column's updates here: sinking them toward their use
+          // would put chains of dependent FMAs on the serial path
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      __syncthreads();
+      if (w == jb) {
+        // the finished columns back to A (off the serial path)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int c = 16 * jb + i;
+          const double v = Lt[c * 64 + l];
+          if (l < n && c < n && c <= l) {
+            if (upper)
+              A[c + (long)l * lda] = v;
+            else
+              A[l + (long)c * lda] = v;
+          }
+        }
+      }
+      if (w > jb) {
+        // rank-16 update: tile q (rows 16q..) -= L[rows][block jb] L[16w..][block jb]^T
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q >= w) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+              const double* col = Lt + (16 * jb + 4 * s + lq) * 64;
+              acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-col[16 * q + li], col[16 * w + li],
+                                                            acc[q], 0, 0, 0);
+            }
           }
         }
       }
     }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int c = 16 * w + i;
-      if (l < n && c < n && c <= l) {
-        if (upper)
-          A[c + (long)l * lda] = a[i];
-        else
-          A[l + (long)c * lda] = a[i];
+    if (w == 0) {
+      // first NaN on the diagonal
+      const unsigned long long bad = __ballot(l < n && isnan(Lt[l * 64 + l]));
+      if (bad && l == 0 && info) {
+        const int j = __ffsll((long long)bad) - 1;
+        const int old = atomicMin(info, goff + j);
+        if ((old < 0 || old > goff + j) && badval) *badval = dv[j];
       }
-      Minv[l + c * LEAF] = m[i];
+      // inverses of the diagonal blocks: lane l -> block b = l/16, column
+      // c = l%16; z = inv(L_bb) e_c by forward substitution
+      const int b = lq, c = li;
+      double z[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) z[q] = q == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        z[q] *= rv[16 * b + q];
+#pragma unroll
+        for (int q2 = q + 1; q2 < 16; ++q2)
+          z[q2] = fma(-Lt[(16 * b + q) * 64 + 16 * b + q2], z[q], z[q2]);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) Minv[b * 256 + q * 16 + c] = z[q];
     }
     __syncthreads();
     if (t == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   // ---- 64 rows below the diagonal block
-  double* Bs = s1;  // Bs[r * CL_LD + k] = B[row0 + r][k]
-  double* Ms = s2;  // Ms[c * CL_LD + k] = M[c][k]
+  double* Bs = s1;  // Bs[r * CL_LD + k] = B[row0 + r][k], later X
+  double* Ls = s2;  // Ls[r * CL_LD + c] = L[r][c] below the diagonal blocks
   const int row0 = (blockIdx.x - 1) * LEAF;
-  for (int idx = t; idx < LEAF * LEAF; idx += CL_T) {
-    const int a0 = idx % LEAF, a1 = idx / LEAF;  // contiguous index first
-    const int r = upper ? a1 : a0, c = upper ? a0 : a1;
-    double v = 0.0;
-    if (row0 + r < mb && c < n)
-      v = upper ? B[c + (long)(row0 + r) * lda] : B[(row0 + r) + (long)c * lda];
-    Bs[r * CL_LD + c] = v;
+  {
+    double v[LEAF * LEAF / CL_T];
+#pragma unroll
+    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+      const int idx = t + q * CL_T;
+      const int a0 = idx % LEAF, a1 = idx / LEAF;  // contiguous index first
+      const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+      v[q] = 0.0;
+      if (row0 + r < mb && c < n)
+        v[q] = upper ? B[c + (long)(row0 + r) * lda] : B[(row0 + r) + (long)c * lda];
+    }
+#pragma unroll
+    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+      const int idx = t + q * CL_T;
+      const int a0 = idx % LEAF, a1 = idx / LEAF;
+      const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+      Bs[r * CL_LD + c] = v[q];
+    }
   }
   if (t == 0) {
     unsigned spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
       if (++spins > LEAF_SPIN_LIMIT) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
+  // The acquire by thread 0 invalidated this CU's caches; after the barrier
+  // every thread's loads see what workgroup 0 released.
   __syncthreads();
-  for (int idx = t; idx < LEAF * LEAF; idx += CL_T)
-    Ms[(idx % LEAF) * CL_LD + idx / LEAF] = ld_agent(&Minv[idx]);
-  __syncthreads();
-  // wave w: X^T rows c = 16w..16w+15 (all 64 rows r of X): D[c][r] =
-  // sum_k M[c][k] B[r][k], k <= 16w+15 (M is lower triangular)
-  v4d acc[4];
+  double mv[4][4];  // inverse blocks in A-operand layout: inv(L_bb)[li][4s + lq]
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
-  const int li = l % 16, lk = l / 16;
-  const int ksteps = 4 * w + 4;
-  for (int s = 0; s < ksteps; ++s) {
-    const int k = 4 * s + lk;
-    const double av = Ms[(16 * w + li) * CL_LD + k];
+  for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double bv = Bs[(16 * q + li) * CL_LD + k];
-      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+    for (int s = 0; s < 4; ++s) mv[b][s] = Minv[b * 256 + li * 16 + 4 * s + lq];
+  {
+    // the factor below the diagonal blocks (the diagonal blocks enter
+    // through their inverses)
+    double v[LEAF * LEAF / CL_T];
+#pragma unroll
+    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+      const int idx = t + q * CL_T;
+      const int a0 = idx % LEAF, a1 = idx / LEAF;
+      const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // L[r][c], contiguous in memory
+      v[q] = 0.0;
+      if (r < n && c < n && (c >> 4) < (r >> 4))
+        v[q] = upper ? A[c + (long)r * lda] : A[r + (long)c * lda];
+    }
+#pragma unroll
+    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+      const int idx = t + q * CL_T;
+      const int a0 = idx % LEAF, a1 = idx / LEAF;
+      const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+      Ls[r * CL_LD + c] = v[q];
     }
   }
   __syncthreads();
-  // D[c][r] at lane l, register e: c = 16w + l/16 + 4e, r = 16q + l%16
+  // wave w: rows R0 = 16w .. 16w+15 of this chunk. Lane l, register e of
+  // xt[b] holds X[R0 + l%16][16b + l/16 + 4e].
+  const int R0 = 16 * w;
+  v4d xt[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int b = 0; b < 4; ++b) {
+    v4d y;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) Bs[(16 * q + li) * CL_LD + 16 * w + lk + 4 * e] = acc[q][e];
+    for (int e = 0; e < 4; ++e) y[e] = Bs[(R0 + li) * CL_LD + 16 * b + lq + 4 * e];
+#pragma unroll
+    for (int a = 0; a < b; ++a)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        // Y^T[c][r] -= sum_k L[16b + c][16a + k] X[r][16a + k]
+        const double lv = -Ls[(16 * b + li) * CL_LD + 16 * a + 4 * s + lq];
+        y = __builtin_amdgcn_mfma_f64_16x16x4f64(lv, xt[a][s], y, 0, 0, 0);
+      }
+    v4d x = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(mv[b][s], y[s], x, 0, 0, 0);
+    xt[b] = x;
+  }
+  // X back to LDS (own rows only), then coalesced stores
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bs[(R0 + li) * CL_LD + 16 * b + lq + 4 * e] = xt[b][e];
   __syncthreads();
   for (int idx = t; idx < LEAF * LEAF; idx += CL_T) {
     const int a0 = idx % LEAF, a1 = idx / LEAF;
@@ -223,7 +334,7 @@ struct LeafFlags {
   int* flags;    // one zeroed flag per leaf launch
   int next;
   int* err;
-  double* minv;  // 64 x 64 inverse of the current leaf's diagonal block
+  double* minv;  // inverses of the current leaf's four 16 x 16 diagonal blocks
 };
 
 int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
@@ -261,7 +372,7 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
 size_t potrf_ws_bytes(int N, int nb) {
   if (nb <= 0 || nb > N) nb = N;
   return (size_t)(ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 8) * sizeof(int) +
-         (size_t)LEAF * LEAF * sizeof(double) + 2048;
+         (size_t)4 * 256 * sizeof(double) + 2048;
 }
 
 namespace {
@@ -310,7 +421,7 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   const int nleaf = ceil_div(N, LEAF) + 2 * ceil_div(N, nb) + 4;
   LeafFlags fl;
   fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
-  fl.minv = (double*)ws_carve((size_t)LEAF * LEAF * sizeof(double));
+  fl.minv = (double*)ws_carve((size_t)4 * 256 * sizeof(double));
   fl.next = 0;
   if (!fl.flags || !fl.minv) {
     set_error("potrf: workspace too small");
