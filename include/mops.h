@@ -134,6 +134,19 @@ int mops_d_dpotrf(int flags, int N, int nb, double *A, int lda, int *info, void 
 int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
                   void *stream);
 
+/* Reverse-mode derivative of DecomposeCHOL(A, LOWER, nb) (no counterpart in
+ * the reference; for callers that differentiate through the factorization).
+ * L: the lower Cholesky factor (N x N, lower triangle read). Lbar: on entry
+ * d loss / dL in its lower triangle (the upper triangle is ignored); on
+ * return the symmetric d loss / dA (both triangles written), so that for a
+ * symmetric perturbation E, d loss = sum_ij Abar_ij E_ij. Blocked reverse
+ * sweep (Murray 2016) with diagonal blocks of order nb (0 or > 64 -> 64).
+ * Uses L, the Lbar buffer and O(64 N) workspace. Host memory, synchronous. */
+int mops_dpotrf_adjoint(const mops_mdata_t *L, mops_mdata_t *Lbar, int N, int nb);
+/* Device-resident twin: Abar holds Lbar on entry. */
+int mops_d_dpotrf_adjoint(int N, int nb, const double *L, int ldl, double *Abar, int lda,
+                          void *stream);
+
 /* ---- LU with partial pivoting -------------------------------------------- */
 
 /* Replaces DecomposeLU / blockedLUpiv / unblockedLUpiv (lu.go:103-273) with

@@ -6,6 +6,6 @@ the host-side mirror of the reference's Go API and the ctypes binding.
 """
 from .matops import (  # noqa: F401
     LEFT, LOWER, NONE, NOTRANS, RIGHT, TRANS, TRANSA, TRANSB, UNIT, UPPER,
-    BlockingParams, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
+    BlockingParams, CholeskyAdjoint, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
     RankUpdateSym, SetPanicOnError, SolveCHOL, SolveLU, SolveTrm,
 )

@@ -37,6 +37,8 @@ SIGNATURES = {
     "mops_dpotrs": (_I, [_PM, _PM, _I, _I, _I]),
     "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),
     "mops_d_dpotrs": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+    "mops_dpotrf_adjoint": (_I, [_PM, _PM, _I, _I]),
+    "mops_d_dpotrf_adjoint": (_I, [_I, _I, _P, _I, _P, _I, _P]),
     "mops_dgetrf": (_I, [_PM, _P, _I, _I, _I, _I]),
     "mops_dgetrs": (_I, [_PM, _PM, _P, _I, _I, _I, _I]),
     "mops_d_dgetrf": (_I, [_I, _I, _I, _P, _I, _P, _P, _P, _P]),

@@ -92,6 +92,41 @@ int mops_dpotrf(mops_mdata_t* A, int flags, int N, int nb) {
   return ST_OK;
 }
 
+int mops_d_dpotrf_adjoint(int N, int nb, const double* L, int ldl, double* Abar, int lda,
+                          void* stream) {
+  clear_error();
+  if (N < 0) return -1;
+  if (nb < 0) return -2;
+  if (ldl < (N > 1 ? N : 1)) return -4;
+  if (lda < (N > 1 ? N : 1)) return -6;
+  if (N == 0) return ST_OK;
+  MOPS_TRY(ws_reserve(chol_adjoint_ws_bytes(N, nb)));
+  return chol_adjoint_d(as_stream(stream), N, nb, L, ldl, Abar, lda);
+}
+
+int mops_dpotrf_adjoint(const mops_mdata_t* L, mops_mdata_t* Lbar, int N, int nb) {
+  clear_error();
+  if (!L || (N > 0 && !L->md)) return -1;
+  if (!Lbar || (N > 0 && !Lbar->md)) return -2;
+  if (N < 0) return -3;
+  if (nb < 0) return -4;
+  if (N == 0) return ST_OK;
+  if (L->step < N) return -1;
+  if (Lbar->step < N) return -2;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)N * N;
+  MOPS_TRY(ws_reserve(2 * na * sizeof(double) + chol_adjoint_ws_bytes(N, nb) + 2048));
+  double* dL = (double*)ws_carve(na * sizeof(double));
+  double* dB = (double*)ws_carve(na * sizeof(double));
+  MOPS_TRY(h2d(dL, N, L->md, L->step, N, N, s));
+  MOPS_TRY(h2d(dB, N, Lbar->md, Lbar->step, N, N, s));
+  MOPS_TRY(chol_adjoint_d(s, N, nb, dL, N, dB, N));
+  MOPS_TRY(d2h(Lbar->md, Lbar->step, dB, N, N, N, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
 int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nrhs) {
   clear_error();
   if (!B || (N > 0 && nrhs > 0 && !B->md)) return -1;

@@ -44,4 +44,10 @@ size_t potrs_ws_bytes(int flags, int N, int nrhs);
 int potrs_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda, double* B,
             long ldb);
 
+// Reverse-mode derivative of the Cholesky factorization: L (lower factor,
+// N x N) and, in Ab, Lbar = d loss / dL (lower triangle read); on return Ab
+// holds the symmetric d loss / dA. Blocked reverse sweep, block nb <= 64.
+size_t chol_adjoint_ws_bytes(int n, int nb);
+int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, double* Ab, long lda);
+
 }  // namespace mops

@@ -185,6 +185,23 @@ def SolveCHOL(B, A, flags):
         ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
 
 
+def CholeskyAdjoint(L, Lbar, nb=64):
+    """Reverse-mode derivative of DecomposeCHOL(A, LOWER, nb) (no counterpart
+    in the reference). L: lower Cholesky factor; Lbar: d loss / dL (lower
+    triangle read). Lbar is overwritten with the symmetric d loss / dA and
+    returned."""
+    n = L.shape[0]
+    if L.shape != (n, n) or Lbar.shape != (n, n):
+        return Lbar, MatopsError("L, Lbar size mismatch")
+    if n == 0:
+        return Lbar, None
+    Lm, _, _ = _view(L)
+    Bm, _, _ = _view(Lbar)
+    check("mops_dpotrf_adjoint", _lib.lib().mops_dpotrf_adjoint(
+        ctypes.byref(Lm), ctypes.byref(Bm), n, nb))
+    return Lbar, None
+
+
 def DecomposeLU(A, pivots, nb):
     """LU with partial pivoting in place (lu.go:257-273). pivots: int64
     numpy array (Go []int), at least min(rows, cols) long. Returns (A, err)."""

@@ -387,5 +387,7 @@ def test_host_api_validation_without_gpu():
     assert str(err) == "A, B size mismatch"
     _, err = mo.DecomposeCHOL(np.zeros((3, 4), order="F"), mo.LOWER, 0)
     assert str(err) == "A not a square matrix"
+    _, err = mo.CholeskyAdjoint(np.zeros((4, 4), order="F"), np.zeros((3, 3), order="F"))
+    assert str(err) == "L, Lbar size mismatch"
     with pytest.raises(ValueError):
         mo.Mult(np.zeros((5, 4)), np.zeros((5, 3)), np.zeros((3, 4)), 1.0, 0.0, 0)  # C order
