@@ -16,16 +16,19 @@
 // so a row interchange moves two contiguous runs of memory, and every row of
 // the panel is a contiguous run. A copy-transpose brings A in and out.
 //
-// Panel (M x w, rows k..m-1): recursive halving down to LEAF_W-wide leaves.
-// A leaf is one cooperative kernel: G workgroups each hold up to 512 rows of
-// the leaf in LDS; per column every workgroup finds its best candidate
-// (wave shuffles), publishes it with the candidate row, and after one grid
-// barrier all of them agree on the pivot, swap the two rows (in LDS for the
-// leaf columns, in global memory for all other columns of A, sliced across
-// the workgroups) and apply the rank-1 update to their rows. Between the
-// halves, the U block is solved with the triangular-inverse TRSM and the
-// lower rows are updated with one GEMM; after the panel the trailing matrix
-// gets the TRSM of U12 and the GEMM A22 -= L21*U12 on the matrix cores.
+// Panel (columns k..k+kb, rows k..m-1), right-looking in 32-column leaves.
+// A leaf is one cooperative kernel: G workgroups of 512 threads, one row
+// per thread held in registers. Per column every workgroup finds its best
+// candidate (wave shuffles), publishes it with the candidate row, and after
+// one grid barrier all of them agree on the pivot, exchange the two rows in
+// registers and apply the rank-1 update to their rows. The interchanges of
+// the leaf's rows in the other panel columns are applied once, at the end
+// of the leaf, as a gather of the rows that moved. After each leaf the
+// U block right of it is solved with the leaf's unit triangle (one small
+// kernel) and the rest of the panel gets one GEMM; after the panel the
+// trailing matrix gets the row interchanges, the TRSM of U12 (by the
+// inverse of the panel's unit triangle) and the GEMM A22 -= L21*U12 on the
+// matrix cores.
 #include <climits>
 #include <cmath>
 
@@ -35,9 +38,10 @@
 namespace mops {
 namespace {
 
-constexpr int LEAF_W = 32;   // leaf width (columns)
-constexpr int LEAF_R = 512;  // rows per workgroup in a leaf (= threads)
+constexpr int LEAF_W = 32;    // leaf width (columns)
+constexpr int LEAF_R = 512;   // rows per workgroup in a leaf (= threads)
 constexpr int LEAF_GMAX = 64;
+constexpr int LEAF_SW = LEAF_W + 3;  // exchange slot: value, row, row data, perm
 constexpr unsigned SPIN_LIMIT = 1u << 26;
 
 // ---- copy-transpose of a column-major tile (64x64 through LDS) ----
@@ -79,19 +83,28 @@ struct LeafArgs {
   int pc0, pc1;    // columns of the enclosing panel (rows swapped in them too)
   int* piv;        // global pivots, piv[c0 + j]
   int* perm;       // perm[i] = original row now at position i
-  double* xb;      // exchange slots: [2][G][LEAF_W + 3]
+  double* xb;      // exchange slots: [2][G][LEAF_SW]
   double* rowk;    // exchange: [2][LEAF_W + 1]
-  unsigned* ctr;   // grid barrier counter (zeroed before launch)
+  double* stage;   // moved values of the other panel columns: [2 * LEAF_W * 256]
+  unsigned* ctr;   // grid barrier counter (monotonic over the factorization)
+  unsigned base;   // its value when this leaf starts
   int* err;        // set to 1 if a barrier wait timed out
 };
 
-// Grid barrier among the workgroups of a cooperative launch.
+// Grid barrier among the workgroups of a cooperative launch. Everything the
+// workgroups exchange inside the leaf is written and read with agent-scope
+// atomics (coherent without cache maintenance), so the barrier only has to
+// order those accesses: every thread's stores are complete (waitcnt) before
+// the workgroup barrier, the arrival is a relaxed atomic add, and the reads
+// that follow the spin depend on it. No L2 write-back or invalidate is
+// needed, which matters while other kernels fill the caches.
 __device__ __forceinline__ void grid_sync(unsigned* ctr, unsigned target, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (++spins > SPIN_LIMIT) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
@@ -102,180 +115,254 @@ __device__ __forceinline__ void grid_sync(unsigned* ctr, unsigned target, int* e
   __syncthreads();
 }
 
-// Pivoted LU of the leaf columns [c0, c0+w) over rows c0..m-1. Each of the G
-// workgroups keeps up to 512 rows of the leaf in LDS, one row per thread.
-// Per column: local best candidate (wave shuffles), publication of the
-// candidate and its row, one grid barrier, then every workgroup reads all
-// candidates at once, agrees on the pivot, swaps the two rows (in LDS for
-// the leaf, in global memory for the other columns of the panel) and updates
-// its rows. Rows of the matrix outside the panel are swapped later, all at
-// once, from the permutation.
+// One DPP lane move of a double (both halves); lanes whose source is out of
+// range keep their own value.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, ROW_MASK,
+                                             BANK_MASK, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, ROW_MASK,
+                                             BANK_MASK, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Wave maximum of v (never NaN) and the lowest lane that holds it. The
+// maximum is a DPP scan (row_shr 1, 2, 3, 4, 8, then row_bcast 15 and 31),
+// which ends in lane 63; the lanes that hold it are found with one ballot.
+// Rows increase with the lane (and with the wave and the workgroup), so the
+// first maximal lane is the reference's tie rule: the lowest row wins.
+__device__ __forceinline__ int wave_first_max(double v, double& vmax) {
+  double m = v;
+  m = fmax(m, dpp_d<0x111, 0xf, 0xf>(m));  // row_shr:1
+  m = fmax(m, dpp_d<0x112, 0xf, 0xf>(m));  // row_shr:2
+  m = fmax(m, dpp_d<0x113, 0xf, 0xf>(m));  // row_shr:3
+  m = fmax(m, dpp_d<0x114, 0xf, 0xe>(m));  // row_shr:4
+  m = fmax(m, dpp_d<0x118, 0xf, 0xc>(m));  // row_shr:8
+  m = fmax(m, dpp_d<0x142, 0xa, 0xf>(m));  // row_bcast:15
+  m = fmax(m, dpp_d<0x143, 0xc, 0xf>(m));  // row_bcast:31
+  const int lo = __builtin_amdgcn_readlane(__double2loint(m), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(m), 63);
+  m = __hiloint2double(hi, lo);
+  vmax = m;
+  return __ffsll((long long)__ballot(v == m)) - 1;
+}
+
+// Pivoted LU of the leaf columns [c0, c0+w) over rows c0..m-1 (the column
+// loop of unblockedLUpiv, lu.go:103-170, with pivotIndex, pivot.go:98-107):
+// per column, the first strictly largest |a| from the diagonal down (a NaN
+// below the diagonal never wins; a NaN on the diagonal is never replaced),
+// the interchange of rows k and p, the column of L scaled by the reciprocal
+// of the pivot and the rank-1 update of the leaf columns right of it.
+// Thread t of workgroup g holds row c0 + 512 g + t of the leaf in
+// registers, rotated: at step j, x[c] is column j + c (mod 32), so the
+// current column is always x[0] and every register index is static while
+// the column loop stays rolled. Rows travel between workgroups in this
+// rotated form, which every thread shares. At the end, the rows that moved
+// are gathered in the other columns of the panel [pc0, pc1) (rows outside
+// the panel move later, all at once, from the permutation).
 __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
-  constexpr int W = LEAF_W, LDW = LEAF_W + 1, SW = LEAF_W + 3;
-  __shared__ double tile[LEAF_R * LDW];
-  __shared__ double xs[LEAF_GMAX * SW];
-  __shared__ double prow[W + 1];
-  __shared__ double okrow[W + 1];
-  __shared__ double wv[LEAF_R / 64];
-  __shared__ int wr[LEAF_R / 64];
-  __shared__ int lperm[LEAF_R];
-  __shared__ int shp;
+  constexpr int W = LEAF_W, SW = LEAF_SW, NWV = LEAF_R / 64;
+  __shared__ double xs[LEAF_GMAX * SW + W + 1];  // slots, then the old row k
+  __shared__ double wv[NWV];
+  __shared__ int wr[NWV];
+  __shared__ double wrow[NWV][W + 1];  // each wave's best row (data, perm)
+  __shared__ double okl[W + 1];        // old row k, when this workgroup holds it
+  __shared__ int pv[W];
+  __shared__ int pair_dst[2 * W], pair_src[2 * W];
 
   const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int r0 = a.c0;
-  const int rg0 = r0 + g * LEAF_R;
-  const int nr = max(0, min(LEAF_R, a.m - rg0));
-  const int w = a.w;
+  const int c0 = a.c0, w = a.w;
+  const int rg0 = c0 + g * LEAF_R;
+  const int row = rg0 + t;
+  const bool mine = row < a.m;
   double* T = a.T;
   const long ldT = a.ldT;
 
-  if (t < nr) {
-    const double* src = T + a.c0 + (long)(rg0 + t) * ldT;
-    for (int c = 0; c < W; ++c) tile[t * LDW + c] = c < w ? src[c] : 0.0;
-    lperm[t] = a.perm[rg0 + t];
-  }
-  __syncthreads();
+  double x[W];
+  int pm = 0;
+#pragma unroll
+  for (int c = 0; c < W; ++c) x[c] = (mine && c < w) ? T[c0 + c + (long)row * ldT] : 0.0;
+  if (mine) pm = a.perm[row];
+  unsigned target = a.base;
 
-  // columns of the panel outside the leaf, sliced across the workgroups
-  const int ncol = (a.pc1 - a.pc0) - w;
-  const int slice = (ncol + G - 1) / G;
+#pragma unroll 1
   for (int j = 0; j < w; ++j) {
-    const int k = r0 + j;
+    const int k = c0 + j;
     double* xbj = a.xb + (long)(j & 1) * G * SW;
     double* rkj = a.rowk + (j & 1) * (W + 1);
-    // 1. local candidate among my rows >= k
+    // 1. candidate of each wave; the winning lane parks its row in LDS.
+    // Keys: |a|; a NaN is -1 (never chosen) except on the diagonal, where
+    // it is +inf (nothing replaces it); rows that are not candidates -2.
     double v = -2.0;
-    int r = INT_MAX;
-    if (t < nr) {
-      const int gr = rg0 + t;
-      if (gr >= k) {
-        const double x = fabs(tile[t * LDW + j]);
-        v = isnan(x) ? (gr == k ? INFINITY : -1.0) : x;
-        r = gr;
-      }
+    if (mine && row >= k) {
+      const double ax = fabs(x[0]);
+      v = isnan(ax) ? (row == k ? INFINITY : -1.0) : ax;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double v2 = __shfl_xor(v, off, 64);
-      const int r2 = __shfl_xor(r, off, 64);
-      if (better(v2, r2, v, r)) {
-        v = v2;
-        r = r2;
-      }
+    double vmax;
+    const int wl = wave_first_max(v, vmax);
+    if (lane == wl) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) wrow[wave][c] = x[c];
+      wrow[wave][W] = (double)pm;
     }
     if (lane == 0) {
-      wv[wave] = v;
-      wr[wave] = r;
+      wv[wave] = vmax;
+      wr[wave] = vmax > -2.0 ? rg0 + 64 * wave + wl : INT_MAX;
+    }
+    if (row == k) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) okl[c] = x[c];
+      okl[W] = (double)pm;
     }
     __syncthreads();
-    if (t == 0) {
-      for (int q = 1; q < LEAF_R / 64; ++q)
-        if (better(wv[q], wr[q], wv[0], wr[0])) {
-          wv[0] = wv[q];
-          wr[0] = wr[q];
-        }
-    }
-    __syncthreads();
-    // 2. publish: value, row, the row's leaf entries and its perm entry
-    {
-      const int br = wr[0];
+    // 2. publish this workgroup's candidate (and the old row k)
+    if (wave == 0) {
+      double bv;
+      const int bq = wave_first_max(lane < NWV ? wv[lane] : -3.0, bv);
+      const int br = wr[bq];
       double* slot = xbj + (long)g * SW;
-      if (t == 0) {
-        st_agent(&slot[0], wv[0]);
+      if (lane == 0) {
+        st_agent(&slot[0], bv);
         st_agent(&slot[1], (double)br);
       }
-      if (br != INT_MAX) {
-        const int lr = br - rg0;
-        if (t < W) st_agent(&slot[2 + t], tile[lr * LDW + t]);
-        if (t == W) st_agent(&slot[2 + W], (double)lperm[lr]);
-      }
-      if (k >= rg0 && k < rg0 + nr) {
-        const int lr = k - rg0;
-        if (t >= 64 && t < 64 + W) st_agent(&rkj[t - 64], tile[lr * LDW + t - 64]);
-        if (t == 64 + W) st_agent(&rkj[W], (double)lperm[lr]);
-      }
+      if (br != INT_MAX && lane <= W) st_agent(&slot[2 + lane], wrow[bq][lane]);
     }
-    grid_sync(a.ctr, (unsigned)(j + 1) * (unsigned)G, a.err);
-    // 3. read every candidate slot and the old row k in one round trip
+    if (wave == 1 && k >= rg0 && k < rg0 + LEAF_R && lane <= W) st_agent(&rkj[lane], okl[lane]);
+    target += (unsigned)G;
+    grid_sync(a.ctr, target, a.err);
+    // 3. every candidate and the old row k, in one round trip
     for (int idx = t; idx < G * SW; idx += LEAF_R) xs[idx] = ld_agent(&xbj[idx]);
-    if (t >= LEAF_R - (W + 1)) okrow[t - (LEAF_R - (W + 1))] = ld_agent(&rkj[t - (LEAF_R - (W + 1))]);
+    if (t >= LEAF_R - (W + 1)) xs[G * SW + t - (LEAF_R - (W + 1))] = ld_agent(&rkj[t - (LEAF_R - (W + 1))]);
     __syncthreads();
-    if (wave == 0) {
-      double bv = -3.0;
-      int br = INT_MAX;
-      for (int q = lane; q < G; q += 64) {
-        const double sv = xs[q * SW];
-        const int sr = (int)xs[q * SW + 1];
-        if (better(sv, sr, bv, br)) {
-          bv = sv;
-          br = sr;
+    // 4. the pivot (every wave agrees on its own): the first workgroup
+    // holding the largest key (workgroups are in row order)
+    double bv;
+    const int bq = wave_first_max(lane < G ? xs[lane * SW] : -3.0, bv);
+    const int br = (int)xs[bq * SW + 1];
+    const int p = bv > -2.0 ? br : k;
+    if (t == 0) {
+      pv[j] = p;
+      if (g == 0) a.piv[k] = p;
+    }
+    const double* prow = xs + bq * SW + 2;  // the pivot row; prow[W] = its perm
+    const double* orow = xs + G * SW;       // the old row k
+    // 5. interchange in registers
+    if (p != k && (row == p || row == k)) {
+      const double* src = row == p ? orow : prow;
+#pragma unroll
+      for (int c = 0; c < W; ++c) x[c] = src[c];
+      pm = (int)src[W];
+    }
+    // 6. column of L and rank-1 update of the rows below k (columns j+1 ..
+    // w-1 are x[1 .. w-1-j]; the rest hold finished columns or padding)
+    if (mine && row > k) {
+      const double l = x[0] * (1.0 / prow[0]);
+      x[0] = l;
+      const int nc = __builtin_amdgcn_readfirstlane(w - 1 - j);
+      // eight columns at a time: the next group's LDS reads wait for this
+      // group's last result, so at most a few rows of prow sit in registers
+      typedef __attribute__((address_space(3))) const double lds_double;
+      lds_double* pr = (lds_double*)prow;
+#pragma unroll
+      for (int c8 = 0; c8 < W; c8 += 8) {
+        asm volatile("" : "+v"(pr) : "v"(x[c8 == 0 ? 0 : c8 - 1]));
+#pragma unroll
+        for (int c = c8; c < c8 + 8; ++c) {
+          if (c == 0) continue;
+          // computed for every c, kept only for the active columns: no
+          // branch, so the LDS reads of a group are issued together
+          const double nv = fma(-l, pr[c], x[c]);
+          x[c] = c <= nc ? nv : x[c];
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(bv, off, 64);
-        const int r2 = __shfl_xor(br, off, 64);
-        if (better(v2, r2, bv, br)) {
-          bv = v2;
-          br = r2;
-        }
-      }
-      if (lane == 0) {
-        const int p = br == INT_MAX ? k : br;
-        shp = p;
-        if (g == 0) a.piv[k] = p;
-      }
     }
-    __syncthreads();
-    const int p = shp;
-    const int gp = (p - r0) / LEAF_R;
-    if (t <= W) prow[t] = xs[gp * SW + 2 + t];
-    __syncthreads();
-    // 4. interchange rows k and p
-    if (p != k) {
-      if (p >= rg0 && p < rg0 + nr && t <= W) {
-        const int lr = p - rg0;
-        if (t < W)
-          tile[lr * LDW + t] = okrow[t];
-        else
-          lperm[lr] = (int)okrow[W];
-      }
-      if (k >= rg0 && k < rg0 + nr && t <= W) {
-        const int lr = k - rg0;
-        if (t < W)
-          tile[lr * LDW + t] = prow[t];
-        else
-          lperm[lr] = (int)prow[W];
-      }
-      // the other columns of the panel, my slice
-      double* Tk = T + (long)k * ldT;
-      double* Tp = T + (long)p * ldT;
-      for (int q = g * slice + t; q < min(ncol, (g + 1) * slice); q += LEAF_R) {
-        int c = a.pc0 + q;
-        if (c >= a.c0) c += w;
-        const double x = Tk[c];
-        Tk[c] = Tp[c];
-        Tp[c] = x;
-      }
-    }
-    __syncthreads();
-    // 5. column of L and rank-1 update of my rows below k
-    if (t < nr) {
-      const int gr = rg0 + t;
-      if (gr > k) {
-        double* row = &tile[t * LDW];
-        const double pinv = 1.0 / prow[j];
-        const double l = row[j] * pinv;
-        row[j] = l;
-        for (int c = j + 1; c < w; ++c) row[c] -= l * prow[c];
-      }
-    }
-    __syncthreads();
+    // 7. rotate: column j moves to the back
+    const double x0 = x[0];
+#pragma unroll
+    for (int c = 0; c < W - 1; ++c) x[c] = x[c + 1];
+    x[W - 1] = x0;
   }
-  if (t < nr) {
-    double* dst = T + a.c0 + (long)(rg0 + t) * ldT;
-    for (int c = 0; c < w; ++c) dst[c] = tile[t * LDW + c];
-    a.perm[rg0 + t] = lperm[t];
+  // after w rotations x[c] holds column (c + w) mod 32
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const int col = (c + w) & (W - 1);
+      if (col < w) T[c0 + col + (long)row * ldT] = x[c];
+    }
+    a.perm[row] = pm;
   }
+  // The other panel columns: position d ends up with the row that was at
+  // src(d), found by running the leaf's interchanges backwards.
+  const int ncol = (a.pc1 - a.pc0) - w;
+  if (ncol <= 0) return;
+  __syncthreads();
+  if (t < 2 * w) {
+    const int d = (t & 1) ? pv[t >> 1] : c0 + (t >> 1);
+    int y = d;
+    for (int jj = w - 1; jj >= 0; --jj) {
+      if (y == c0 + jj)
+        y = pv[jj];
+      else if (y == pv[jj])
+        y = c0 + jj;
+    }
+    pair_dst[t] = d;
+    pair_src[t] = y;
+  }
+  __syncthreads();
+  // stage every moved value (agent-scope atomics, visible to all
+  // workgroups), wait for all of them, then write the destinations
+  const int items = 2 * w * ncol;
+  const int stride = G * LEAF_R;
+  for (int i2 = g * LEAF_R + t; i2 < items; i2 += stride) {
+    const int pq = i2 / ncol;
+    int c = a.pc0 + i2 % ncol;
+    if (c >= c0) c += w;
+    if (pair_src[pq] != pair_dst[pq]) st_agent(&a.stage[i2], T[c + (long)pair_src[pq] * ldT]);
+  }
+  target += (unsigned)G;
+  grid_sync(a.ctr, target, a.err);
+  for (int i2 = g * LEAF_R + t; i2 < items; i2 += stride) {
+    const int pq = i2 / ncol;
+    int c = a.pc0 + i2 % ncol;
+    if (c >= c0) c += w;
+    if (pair_src[pq] != pair_dst[pq]) T[c + (long)pair_dst[pq] * ldT] = ld_agent(&a.stage[i2]);
+  }
+}
+
+// U block right of a leaf: for each column q of A in [c0+w, c1), solve
+// L11 u = a (L11 unit lower, the leaf's w x w diagonal block; a = A[c0:c0+w,
+// q]) in place. Thread = column; T layout (A[r][q] = T[q + r*ldT]).
+__global__ __launch_bounds__(256) void k_unit_lower_solve(double* T, long ldT, int c0, int w,
+                                                          int c1) {
+  constexpr int W = LEAF_W;
+  __shared__ double Ls[W][W + 1];  // Ls[i][j] = L11[i][j]
+  for (int idx = threadIdx.x; idx < W * W; idx += 256) {
+    const int i = idx / W, jj = idx % W;
+    Ls[i][jj] = (i < w && jj < i) ? T[(c0 + jj) + (long)(c0 + i) * ldT] : 0.0;
+  }
+  __syncthreads();
+  const int q = c0 + w + blockIdx.x * 256 + threadIdx.x;
+  if (q >= c1) return;
+  double u[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const double v = T[q + (long)(c0 + min(i, w - 1)) * ldT];  // in bounds for any i
+    u[i] = i < w ? v : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    // make this step's LDS reads wait for u[i]: hoisting all 496 of them
+    // ahead of the arithmetic would need twice the register file
+    typedef __attribute__((address_space(3))) const double lds_double;
+    lds_double* col = (lds_double*)&Ls[0][i];
+    asm volatile("" : "+v"(col) : "v"(u[i]));
+#pragma unroll
+    for (int i2 = i + 1; i2 < W; ++i2) u[i2] = fma(-col[i2 * (W + 1)], u[i], u[i2]);
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+    if (i < w) T[q + (long)(c0 + i) * ldT] = u[i];
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -401,7 +488,9 @@ struct LuCtx {
   double* tmp[2];   // U12 staging for the two streams
   double* xb;
   double* rowk;
+  double* stage;
   unsigned* ctr;
+  unsigned sync_base;  // barrier arrivals queued so far on ctr
   int* err;
 };
 
@@ -412,8 +501,10 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
     set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
     return ST_BADARG;
   }
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
-  MOPS_CHECK_HIP(hipGetLastError());
+  if (pc1 - pc0 > 256) {
+    set_error("getrf: panel wider than 256 columns");
+    return ST_BADARG;
+  }
   LeafArgs a;
   a.T = c.T;
   a.ldT = c.ldT;
@@ -426,33 +517,39 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   a.perm = c.perm;
   a.xb = c.xb;
   a.rowk = c.rowk;
+  a.stage = c.stage;
   a.ctr = c.ctr;
+  a.base = c.sync_base;
   a.err = c.err;
   void* args[] = {&a};
   MOPS_CHECK_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_leaf, dim3(G), dim3(LEAF_R),
                                             args, 0, s));
+  // one barrier per column, plus one for the gather when the panel is wider
+  const int nsync = w + ((pc1 - pc0) - w > 0 ? 1 : 0);
+  c.sync_base += (unsigned)nsync * (unsigned)G;
   return ST_OK;
 }
 
-// recursive factorization of the panel columns [c0, c0+w) inside the panel
-// [pc0, pc1), rows c0..m-1
-int panel_rec(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
-  if (w <= LEAF_W) return leaf(s, c, c0, w, pc0, pc1);
-  int h = ((w / 2 + LEAF_W - 1) / LEAF_W) * LEAF_W;
-  if (h >= w) h = w - LEAF_W;
-  MOPS_TRY(panel_rec(s, c, c0, h, pc0, pc1));
+// Panel columns [pc0, pc1), rows pc0..m-1: leaves of LEAF_W columns; after
+// each, the U block right of it (unit lower solve) and one GEMM for the
+// rest of the panel.
+int panel_factor(hipStream_t s, LuCtx& c, int pc0, int pc1) {
   const long ld = c.ldT;
   double* T = c.T;
-  // U block of the right half: A[c0:c0+h, c0+h:c0+w] = L11^-1 (.), i.e. in
-  // T: X <- X * (L11^T)^-1, L11^T unit upper in T
-  MOPS_TRY(trsm_d(s, F_RIGHT | F_UPPER | F_UNIT, h, w - h, 1.0, T + c0 + (long)c0 * ld, ld,
-                  T + (c0 + h) + (long)c0 * ld, ld));
-  const int below = c.m - c0 - h;
-  if (below > 0)
-    MOPS_TRY(gemm_d(s, false, false, w - h, below, h, -1.0, T + (c0 + h) + (long)c0 * ld, ld,
-                    T + c0 + (long)(c0 + h) * ld, ld, 1.0, T + (c0 + h) + (long)(c0 + h) * ld,
-                    ld));
-  if (c0 + h < c.m) MOPS_TRY(panel_rec(s, c, c0 + h, w - h, pc0, pc1));
+  for (int c0 = pc0; c0 < pc1; c0 += LEAF_W) {
+    const int w = min(LEAF_W, pc1 - c0);
+    MOPS_TRY(leaf(s, c, c0, w, pc0, pc1));
+    const int right = pc1 - (c0 + w);
+    if (right <= 0) break;
+    hipLaunchKernelGGL(k_unit_lower_solve, dim3(ceil_div(right, 256)), dim3(256), 0, s, T, ld, c0,
+                       w, pc1);
+    MOPS_CHECK_HIP(hipGetLastError());
+    const int below = c.m - (c0 + w);
+    if (below > 0)
+      MOPS_TRY(gemm_d(s, false, false, right, below, w, -1.0, T + (c0 + w) + (long)c0 * ld, ld,
+                      T + c0 + (long)(c0 + w) * ld, ld, 1.0, T + (c0 + w) + (long)(c0 + w) * ld,
+                      ld));
+  }
   return ST_OK;
 }
 
@@ -463,7 +560,7 @@ int panel(hipStream_t s, LuCtx& c, int k, int kb, int par) {
   hipLaunchKernelGGL(k_inv_perm, dim3(grid1(rows)), dim3(256), 0, s, c.perm, k, c.m, c.inv0);
   MOPS_CHECK_HIP(hipGetLastError());
   trace_mark_begin(s, TR_PANEL);
-  MOPS_TRY(panel_rec(s, c, k, kb, k, k + kb));
+  MOPS_TRY(panel_factor(s, c, k, k + kb));
   trace_mark_end(s, TR_PANEL, 0.0);
   hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, c.pairs[par]);
   hipLaunchKernelGGL(k_perm_pairs, dim3(grid1(rows)), dim3(256), 0, s, c.perm, c.inv0, k, c.m,
@@ -516,6 +613,7 @@ size_t getrf_ws_bytes(int m, int n, int nb) {
   const int G = ceil_div(m, LEAF_R);
   size_t b = (size_t)m * n * sizeof(double);                   // T
   b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;   // exchange
+  b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
   b += (size_t)(mn + 3 * m + 64) * sizeof(int);                // piv, perm, inv0, ctr, err
   b += (size_t)2 * (4 * nb + 2) * sizeof(int);                 // pairs
   b += (size_t)2 * nb * nb * sizeof(double);                   // linv
@@ -546,6 +644,7 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   c.T = (double*)ws_carve((size_t)m * n * sizeof(double));
   c.xb = (double*)ws_carve((size_t)2 * G * (LEAF_W + 3) * sizeof(double));
   c.rowk = (double*)ws_carve(2 * (LEAF_W + 1) * sizeof(double));
+  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
   c.piv = (int*)ws_carve((size_t)mn * sizeof(int));
   c.perm = (int*)ws_carve((size_t)m * sizeof(int));
   c.inv0 = (int*)ws_carve((size_t)m * sizeof(int));
@@ -557,12 +656,14 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   c.tmp[1] = (double*)ws_carve((size_t)n * nb * sizeof(double));
   c.ctr = (unsigned*)ws_carve(256);
   c.err = err_flag ? err_flag : (int*)ws_carve(256);
-  if (!c.T || !c.xb || !c.rowk || !c.piv || !c.perm || !c.inv0 || !c.pairs[0] || !c.pairs[1] ||
+  if (!c.T || !c.xb || !c.rowk || !c.stage || !c.piv || !c.perm || !c.inv0 || !c.pairs[0] || !c.pairs[1] ||
       !c.linv[0] || !c.linv[1] || !c.tmp[0] || !c.tmp[1] || !c.ctr || !c.err) {
     set_error("getrf: workspace too small");
     return ST_NOMEM;
   }
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.err, 1, 0);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
+  c.sync_base = 0;
   hipLaunchKernelGGL(k_iota, dim3(grid1(m)), dim3(256), 0, s, c.perm, m);
   // T = A^T
   hipLaunchKernelGGL((k_transpose<double, double>), dim3(ceil_div(m, 64), ceil_div(n, 64)),
