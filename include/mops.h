@@ -177,6 +177,34 @@ int mops_d_dgetrf(int M, int N, int nb, double *A, int lda, int64_t *ipiv, int *
 int mops_d_dgetrs(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,
                   int npiv, const int *perm, double *B, int ldb, void *stream);
 
+/* ---- Mixed precision LU with iterative refinement ------------------------ */
+
+/* No counterpart in the reference: for callers that want the LU solve of a
+ * well-conditioned system faster than the FP64 factorization allows. */
+enum { MOPS_PREC_FP64 = 0, MOPS_PREC_FP32 = 1 };
+
+typedef struct {
+  int iterations;        /* solves with the factors (1 = no refinement step) */
+  int converged;         /* 1 if backward_error <= tol */
+  double backward_error; /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) */
+  double first_error;    /* the same after the first solve */
+  double factor_ms;      /* device time of the factorization */
+  double refine_ms;      /* device time of the solves and residuals */
+} mops_ir_report;
+
+/* X (N x nrhs) = A^-1 B: LU with partial pivoting in precision prec
+ * (MOPS_PREC_FP32: FP32 trailing updates on the FP32 matrix cores, FP64
+ * panels; MOPS_PREC_FP64: the DecomposeLU path), then FP64 refinement
+ * X += solve(B - A X) until backward_error <= tol or maxit solves. A and B
+ * are not modified. Returns 0 even when not converged (see rep->converged).
+ * Host memory, synchronous. */
+int mops_dgesv_ir(const mops_mdata_t *A, const mops_mdata_t *B, mops_mdata_t *X, int N, int nrhs,
+                  int prec, double tol, int maxit, mops_ir_report *rep);
+/* Device-resident twin (rep is a host pointer; the call synchronizes the
+ * stream once per refinement step). */
+int mops_d_dgesv_ir(int N, int nrhs, const double *A, int lda, const double *B, int ldb, double *X,
+                    int ldx, int prec, double tol, int maxit, mops_ir_report *rep, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,5 +7,5 @@ the host-side mirror of the reference's Go API and the ctypes binding.
 from .matops import (  # noqa: F401
     LEFT, LOWER, NONE, NOTRANS, RIGHT, TRANS, TRANSA, TRANSB, UNIT, UPPER,
     BlockingParams, CholeskyAdjoint, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
-    RankUpdateSym, SetPanicOnError, SolveCHOL, SolveLU, SolveTrm,
+    PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
 )

@@ -20,6 +20,15 @@ _I = ctypes.c_int
 _D = ctypes.c_double
 _PM = ctypes.POINTER(MData)
 
+class IrReport(ctypes.Structure):
+    """mops_ir_report."""
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("backward_error", ctypes.c_double), ("first_error", ctypes.c_double),
+                ("factor_ms", ctypes.c_double), ("refine_ms", ctypes.c_double)]
+
+
+_PR = ctypes.POINTER(IrReport)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mops_last_error": (ctypes.c_char_p, []),
@@ -38,6 +47,8 @@ SIGNATURES = {
     "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),
     "mops_d_dpotrs": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
     "mops_dpotrf_adjoint": (_I, [_PM, _PM, _I, _I]),
+    "mops_dgesv_ir": (_I, [_PM, _PM, _PM, _I, _I, _I, _D, _I, _PR]),
+    "mops_d_dgesv_ir": (_I, [_I, _I, _P, _I, _P, _I, _P, _I, _I, _D, _I, _PR, _P]),
     "mops_d_dpotrf_adjoint": (_I, [_I, _I, _P, _I, _P, _I, _P]),
     "mops_dgetrf": (_I, [_PM, _P, _I, _I, _I, _I]),
     "mops_dgetrs": (_I, [_PM, _PM, _P, _I, _I, _I, _I]),

@@ -249,3 +249,60 @@ int mops_dgetrs(mops_mdata_t* B, const mops_mdata_t* A, const int64_t* pivots, i
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Mixed precision LU with iterative refinement
+
+extern "C" {
+
+int mops_d_dgesv_ir(int N, int nrhs, const double* A, int lda, const double* B, int ldb, double* X,
+                    int ldx, int prec, double tol, int maxit, mops_ir_report* rep, void* stream) {
+  clear_error();
+  if (N < 0) return -1;
+  if (nrhs < 0) return -2;
+  if (lda < (N > 1 ? N : 1)) return -4;
+  if (ldb < (N > 1 ? N : 1)) return -6;
+  if (ldx < (N > 1 ? N : 1)) return -8;
+  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32) {
+    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32)", prec);
+    return -9;
+  }
+  if (!rep) return -12;
+  MOPS_TRY(ws_reserve(gesv_ir_ws_bytes(N, nrhs, prec)));
+  return gesv_ir_d(as_stream(stream), N, nrhs, A, lda, B, ldb, X, ldx, prec, tol, maxit, rep);
+}
+
+int mops_dgesv_ir(const mops_mdata_t* A, const mops_mdata_t* B, mops_mdata_t* X, int N, int nrhs,
+                  int prec, double tol, int maxit, mops_ir_report* rep) {
+  clear_error();
+  if (!A || (N > 0 && !A->md)) return -1;
+  if (!B || (N > 0 && nrhs > 0 && !B->md)) return -2;
+  if (!X || (N > 0 && nrhs > 0 && !X->md)) return -3;
+  if (N < 0) return -4;
+  if (nrhs < 0) return -5;
+  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32) {
+    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32)", prec);
+    return -6;
+  }
+  if (!rep) return -9;
+  if (N == 0 || nrhs == 0) {
+    *rep = mops_ir_report{0, 1, 0.0, 0.0, 0.0, 0.0};
+    return ST_OK;
+  }
+  if (A->step < N || B->step < N || X->step < N) return -1;
+  hipStream_t s = need_stream();
+  if (!s) return ST_HIP;
+  const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  MOPS_TRY(ws_reserve((na + 2 * nb) * sizeof(double) + gesv_ir_ws_bytes(N, nrhs, prec) + 4096));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  double* dB = (double*)ws_carve(nb * sizeof(double));
+  double* dX = (double*)ws_carve(nb * sizeof(double));
+  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
+  MOPS_TRY(h2d(dB, N, B->md, B->step, N, nrhs, s));
+  MOPS_TRY(gesv_ir_d(s, N, nrhs, dA, N, dB, N, dX, N, prec, tol, maxit, rep));
+  MOPS_TRY(d2h(X->md, X->step, dX, N, N, nrhs, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// Solve A X = B by LU in reduced precision plus FP64 iterative refinement
+// (no counterpart in matops; it reuses the LU path's kernels).
+//
+//   factor:  prec FP32 -> getrf_mixed_d (FP32 trailing updates, FP64 panels)
+//            prec FP64 -> getrf_d (the plain FP64 path; one solve suffices)
+//   loop:    D = solve(LU, R)   (FP64 triangular solves on the factors)
+//            X += D
+//            R = B - A X        (FP64 GEMM with the original A)
+//            eta = max_j ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+//   until eta <= tol or maxit solves.
+#include <cstring>
+
+#include "../../include/mops.h"
+#include "mops_lu.hpp"
+#include "mops_solvers.hpp"
+
+namespace mops {
+namespace {
+
+// non-negative doubles order like their bit patterns
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// out = max_i sum_j |A[i][j]| (the infinity norm); thread = row
+__global__ void k_inf_norm(int n, const double* __restrict__ A, long lda,
+                           unsigned long long* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += fabs(A[i + (long)j * lda]);
+  atomic_max_nonneg(out, s);
+}
+
+// out[j] = max_i |M[i][j]|; workgroup = column
+__global__ __launch_bounds__(256) void k_col_absmax(int n, const double* __restrict__ M, long ld,
+                                                    double* out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(M[i + (long)j * ld]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+__global__ void k_add_tile(int n, int nrhs, const double* __restrict__ D, long ldd, double* X,
+                           long ldx) {
+  const long total = (long)n * nrhs;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % n), j = (int)(t / n);
+    X[i + (long)j * ldx] += D[i + (long)j * ldd];
+  }
+}
+
+__global__ void k_zero_tile(int n, int nrhs, double* X, long ldx) {
+  const long total = (long)n * nrhs;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x)
+    X[(t % n) + (t / n) * ldx] = 0.0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// eta = max_j rmax_j / (anorm xmax_j + bmax_j)
+__global__ void k_eta(int nrhs, const double* rmax, const double* xmax, const double* bmax,
+                      const unsigned long long* anorm_bits, double* eta) {
+  if (threadIdx.x != 0) return;
+  const double an = __longlong_as_double((long long)*anorm_bits);
+  double e = 0.0;
+  for (int j = 0; j < nrhs; ++j) {
+    const double den = an * xmax[j] + bmax[j];
+    const double q = den > 0.0 ? rmax[j] / den : (rmax[j] > 0.0 ? INFINITY : 0.0);
+    e = fmax(e, q);
+  }
+  *eta = e;
+}
+
+inline int grid_for(long total) {
+  long b = (total + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+}  // namespace
+
+size_t gesv_ir_ws_bytes(int n, int nrhs, int prec) {
+  if (n <= 0 || nrhs <= 0) return 0;
+  size_t b = (size_t)n * n * sizeof(double);          // LU
+  b += 2 * (size_t)n * nrhs * sizeof(double);         // R, D
+  b += (size_t)n * sizeof(int) + 4 * nrhs * sizeof(double) + 4096;
+  const size_t f = prec == 1 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
+  const size_t r = getrs_ws_bytes(n, nrhs);
+  return b + (f > r ? f : r);
+}
+
+int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const double* B,
+              long ldb, double* X, long ldx, int prec, double tol, int maxit,
+              mops_ir_report* res) {
+  std::memset(res, 0, sizeof(*res));
+  if (n <= 0 || nrhs <= 0) {
+    res->converged = 1;
+    return ST_OK;
+  }
+  if (prec != 0 && prec != 1) {
+    set_error("gesv_ir: precision %d is not available (0 = FP64, 1 = FP32)", prec);
+    return ST_BADARG;
+  }
+  if (maxit < 1) maxit = 1;
+  double* LU = (double*)ws_carve((size_t)n * n * sizeof(double));
+  double* R = (double*)ws_carve((size_t)n * nrhs * sizeof(double));
+  double* D = (double*)ws_carve((size_t)n * nrhs * sizeof(double));
+  int* perm = (int*)ws_carve((size_t)n * sizeof(int));
+  double* cm = (double*)ws_carve(4 * (size_t)nrhs * sizeof(double) + 64);
+  unsigned long long* anorm = (unsigned long long*)ws_carve(64);
+  int* err = (int*)ws_carve(64);
+  if (!LU || !R || !D || !perm || !cm || !anorm || !err) {
+    set_error("gesv_ir: workspace too small");
+    return ST_NOMEM;
+  }
+  double *rmax = cm, *xmax = cm + nrhs, *bmax = cm + 2 * nrhs, *eta = cm + 3 * nrhs;
+  // own events: the factorizations use the shared synchronization events
+  struct Events {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (hipEvent_t& x : ev.e) MOPS_CHECK_HIP(hipEventCreate(&x));
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
+  MOPS_CHECK_HIP(hipEventRecord(e0, s));
+  {
+    const size_t mark = ws_mark();
+    if (prec == 1) {
+      MOPS_TRY(getrf_mixed_d(s, n, 256, A, lda, LU, n, nullptr, perm, err));
+    } else {
+      MOPS_TRY(copy_tile_d(s, n, n, A, lda, LU, n));
+      MOPS_TRY(getrf_d(s, n, n, 256, LU, n, nullptr, perm, err));
+    }
+    ws_release(mark);
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e1, s));
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, anorm);
+  hipLaunchKernelGGL(k_inf_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, A, lda, anorm);
+  hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, B, ldb, bmax);
+  hipLaunchKernelGGL(k_zero_tile, dim3(grid_for((long)n * nrhs)), dim3(256), 0, s, n, nrhs, X, ldx);
+  MOPS_CHECK_HIP(hipGetLastError());
+  MOPS_TRY(copy_tile_d(s, n, nrhs, B, ldb, R, n));
+  for (int it = 1; it <= maxit; ++it) {
+    MOPS_TRY(copy_tile_d(s, n, nrhs, R, n, D, n));
+    {
+      const size_t mark = ws_mark();
+      const int st = getrs_d(s, 0, n, nrhs, LU, n, nullptr, 0, perm, D, n);
+      ws_release(mark);
+      MOPS_TRY(st);
+    }
+    hipLaunchKernelGGL(k_add_tile, dim3(grid_for((long)n * nrhs)), dim3(256), 0, s, n, nrhs, D,
+                       (long)n, X, ldx);
+    MOPS_TRY(copy_tile_d(s, n, nrhs, B, ldb, R, n));
+    MOPS_TRY(gemm_d(s, false, false, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, n));
+    hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, R, (long)n, rmax);
+    hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, X, ldx, xmax);
+    hipLaunchKernelGGL(k_eta, dim3(1), dim3(1), 0, s, nrhs, rmax, xmax, bmax, anorm, eta);
+    MOPS_CHECK_HIP(hipGetLastError());
+    double e = 0.0;
+    MOPS_CHECK_HIP(hipMemcpyAsync(&e, eta, sizeof(double), hipMemcpyDeviceToHost, s));
+    MOPS_CHECK_HIP(hipStreamSynchronize(s));
+    res->iterations = it;
+    res->backward_error = e;
+    if (it == 1) res->first_error = e;
+    if (e <= tol) {
+      res->converged = 1;
+      break;
+    }
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e2, s));
+  MOPS_CHECK_HIP(hipEventSynchronize(e2));
+  float f_ms = 0.f, r_ms = 0.f;
+  MOPS_CHECK_HIP(hipEventElapsedTime(&f_ms, e0, e1));
+  MOPS_CHECK_HIP(hipEventElapsedTime(&r_ms, e1, e2));
+  res->factor_ms = f_ms;
+  res->refine_ms = r_ms;
+  int eflag = 0;
+  MOPS_CHECK_HIP(hipMemcpy(&eflag, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (eflag) {
+    set_error("gesv_ir: LU panel grid barrier timed out");
+    return ST_HIP;
+  }
+  return ST_OK;
+}
+
+}  // namespace mops

@@ -404,9 +404,11 @@ __global__ void k_perm_pairs(const int* perm, const int* inv0, int k, int m, int
 // columns [c_lo, c_hi) of A, i.e. T-rows: every workgroup takes 16 of them,
 // reads all source values into LDS, then writes the destinations.
 constexpr int AP_ROWS = 16;
-__global__ __launch_bounds__(256) void k_apply_pairs(double* T, long ldT, const int* pairs,
+template <typename D>
+__global__ __launch_bounds__(256) void k_apply_pairs(D* T, long ldT, const int* pairs,
                                                      int maxpairs, int c_lo, int c_hi) {
-  extern __shared__ double vals[];  // [maxpairs][AP_ROWS]
+  extern __shared__ double vals_raw[];  // [maxpairs][AP_ROWS]
+  D* vals = reinterpret_cast<D*>(vals_raw);
   const int np = min(pairs[0], maxpairs);
   const int cbase = c_lo + blockIdx.x * AP_ROWS;
   const int nc = min(AP_ROWS, c_hi - cbase);
@@ -575,8 +577,8 @@ int panel(hipStream_t s, LuCtx& c, int k, int kb, int par) {
 int apply_rows(hipStream_t s, LuCtx& c, int par, int c_lo, int c_hi) {
   if (c_hi <= c_lo) return ST_OK;
   const size_t shm = (size_t)c.maxpairs * AP_ROWS * sizeof(double);
-  hipLaunchKernelGGL(k_apply_pairs, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm, s, c.T,
-                     c.ldT, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  hipLaunchKernelGGL(k_apply_pairs<double>, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm,
+                     s, c.T, c.ldT, c.pairs[par], c.maxpairs, c_lo, c_hi);
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }
@@ -718,6 +720,183 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
     MOPS_CHECK_HIP(hipGetLastError());
   }
   if (perm_out) MOPS_CHECK_HIP(hipMemcpyAsync(perm_out, c.perm, (size_t)m * sizeof(int),
+                                              hipMemcpyDeviceToDevice, s));
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: the trailing matrix in FP32, the panels in FP64.
+//
+// T32 = A^T in FP32 holds the matrix between panels; every trailing update
+// (row interchanges, U12 by the inverse of the panel's unit triangle, and
+// A22 -= L21 U12) runs on FP32 data with the FP32 matrix cores, at twice the
+// FP64 rate and half the bytes. Each panel is widened to FP64 and factored
+// by the same code as getrf_d (pivot search, interchanges and the panel's
+// own updates in FP64), then narrowed back. The factors are returned in
+// FP64 (exactly the FP32 values) for the refinement solves.
+
+namespace {
+
+struct MixedCtx {
+  float* T32;        // n x n, T32[c + r*n] = A[r][c]
+  float* linv32[2];  // FP32 copies of the panels' unit-triangle inverses
+  float* tmp32[2];   // U12 staging for the two streams
+  double* P64;       // one FP64 panel: nb x n, P64[(c-k) + (r-k)*nb]
+};
+
+int apply_rows32(hipStream_t s, LuCtx& c, MixedCtx& x, int par, int c_lo, int c_hi) {
+  if (c_hi <= c_lo) return ST_OK;
+  const size_t shm = (size_t)c.maxpairs * AP_ROWS * sizeof(float);
+  hipLaunchKernelGGL(k_apply_pairs<float>, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm,
+                     s, x.T32, (long)c.n, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int update_cols32(hipStream_t s, LuCtx& c, MixedCtx& x, int k, int kb, int par, int j0, int j1,
+                  float* tmp, bool gemm_trace) {
+  if (j1 <= j0) return ST_OK;
+  MOPS_TRY(apply_rows32(s, c, x, par, j0, j1));
+  const long ld = c.n;
+  float* T = x.T32;
+  const int nc = j1 - j0;
+  float* U = T + j0 + (long)k * ld;
+  MOPS_TRY(copy_tile_s(s, nc, kb, U, ld, tmp, nc));
+  MOPS_TRY(gemm_s(s, false, false, nc, kb, kb, 1.0f, tmp, nc, x.linv32[par], kb, 0.0f, U, ld,
+                  GM_FULL, TRI_B_UPPER));
+  const int below = c.m - k - kb;
+  if (below <= 0) return ST_OK;
+  if (gemm_trace) trace_mark_begin(s, TR_GEMM_UPDATE);
+  MOPS_TRY(gemm_s(s, false, false, nc, below, kb, -1.0f, U, ld, T + k + (long)(k + kb) * ld, ld,
+                  1.0f, T + j0 + (long)(k + kb) * ld, ld));
+  if (gemm_trace) trace_mark_end(s, TR_GEMM_UPDATE, 2.0 * nc * (double)below * kb);
+  return ST_OK;
+}
+
+// Panel [k, k+kb) of T32: widen, factor in FP64 (panel() on a context whose
+// T is the FP64 panel), narrow back; FP32 copy of the triangle's inverse.
+int panel_mixed(hipStream_t s, LuCtx& c, MixedCtx& x, int k, int kb, int par) {
+  const long ld = c.n;
+  const int rows = c.m - k;
+  MOPS_TRY(convert_s2d(s, kb, rows, x.T32 + k + (long)k * ld, ld, x.P64, kb));
+  LuCtx pc = c;
+  pc.T = x.P64 - k - (long)k * kb;  // only columns k..k+kb, rows k..m are touched
+  pc.ldT = kb;
+  MOPS_TRY(panel(s, pc, k, kb, par));
+  c.sync_base = pc.sync_base;
+  MOPS_TRY(convert_d2s(s, kb, kb, c.linv[par], kb, x.linv32[par], kb));
+  return convert_d2s(s, kb, rows, x.P64, kb, x.T32 + k + (long)k * ld, ld);
+}
+
+}  // namespace
+
+size_t getrf_mixed_ws_bytes(int n, int nb) {
+  if (n <= 0) return 0;
+  if (nb <= 0 || nb > 256) nb = 256;
+  if (nb > n) nb = n;
+  const int G = ceil_div(n, LEAF_R);
+  size_t b = (size_t)n * n * sizeof(float);                    // T32
+  b += (size_t)nb * n * sizeof(double);                        // P64
+  b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;   // exchange
+  b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
+  b += (size_t)(4 * n + 64) * sizeof(int);                     // piv, perm, inv0, ctr, err
+  b += (size_t)2 * (4 * nb + 2) * sizeof(int);                 // pairs
+  b += (size_t)2 * nb * nb * (sizeof(double) + sizeof(float));  // linv
+  b += (size_t)2 * n * nb * sizeof(float);                     // tmp32
+  b += trtri_ws_bytes(nb) + 32 * 256;
+  return b;
+}
+
+int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, double* LU, long ldlu,
+                  long long* ipiv, int* perm_out, int* err_flag) {
+  if (n <= 0) return ST_OK;
+  if (nb <= 0 || nb > 256) nb = 256;
+  if (nb > n) nb = n;
+  const int G = ceil_div(n, LEAF_R);
+  if (G > LEAF_GMAX) {
+    set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
+    return ST_BADARG;
+  }
+  LuCtx c;
+  MixedCtx x;
+  c.m = n;
+  c.n = n;
+  c.ldT = n;
+  c.T = nullptr;
+  c.maxpairs = 2 * nb;
+  x.T32 = (float*)ws_carve((size_t)n * n * sizeof(float));
+  x.P64 = (double*)ws_carve((size_t)nb * n * sizeof(double));
+  c.xb = (double*)ws_carve((size_t)2 * G * (LEAF_W + 3) * sizeof(double));
+  c.rowk = (double*)ws_carve(2 * (LEAF_W + 1) * sizeof(double));
+  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
+  c.piv = (int*)ws_carve((size_t)n * sizeof(int));
+  c.perm = (int*)ws_carve((size_t)n * sizeof(int));
+  c.inv0 = (int*)ws_carve((size_t)n * sizeof(int));
+  for (int q = 0; q < 2; ++q) {
+    c.pairs[q] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
+    c.linv[q] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
+    x.linv32[q] = (float*)ws_carve((size_t)nb * nb * sizeof(float));
+    x.tmp32[q] = (float*)ws_carve((size_t)n * nb * sizeof(float));
+    c.tmp[q] = nullptr;
+  }
+  c.ctr = (unsigned*)ws_carve(256);
+  c.err = err_flag ? err_flag : (int*)ws_carve(256);
+  if (!x.T32 || !x.P64 || !c.xb || !c.rowk || !c.stage || !c.piv || !c.perm || !c.inv0 ||
+      !c.pairs[0] || !c.pairs[1] || !c.linv[0] || !c.linv[1] || !x.linv32[0] || !x.linv32[1] ||
+      !x.tmp32[0] || !x.tmp32[1] || !c.ctr || !c.err) {
+    set_error("getrf_mixed: workspace too small");
+    return ST_NOMEM;
+  }
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.err, 1, 0);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
+  c.sync_base = 0;
+  hipLaunchKernelGGL(k_iota, dim3(grid1(n)), dim3(256), 0, s, c.perm, n);
+  hipLaunchKernelGGL((k_transpose<double, float>), dim3(ceil_div(n, 64), ceil_div(n, 64)),
+                     dim3(256), 0, s, n, n, A, lda, x.T32, (long)n);
+  MOPS_CHECK_HIP(hipGetLastError());
+
+  // the look-ahead schedule of getrf_d, on FP32 trailing data
+  hipStream_t s2 = aux_stream();
+  hipEvent_t e_fork = sync_event(0), e_pan = sync_event(1), e_rest = sync_event(2),
+             e_join = sync_event(3);
+  if (!s2 || !e_fork || !e_pan || !e_rest || !e_join) {
+    set_error("getrf_mixed: could not create the look-ahead stream");
+    return ST_HIP;
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
+  MOPS_TRY(panel_mixed(s2, c, x, 0, nb, 0));
+  MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
+  bool have_rest = false;
+  int par = 0;
+  for (int k = 0; k < n; k += nb, par ^= 1) {
+    const int kb = min(nb, n - k);
+    const int kb1 = min(nb, n - k - kb);
+    const int j_next = k + kb, j_rest = k + kb + kb1;
+    if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
+    MOPS_TRY(apply_rows32(s, c, x, par, 0, k));
+    if (j_rest < n) {
+      MOPS_TRY(update_cols32(s, c, x, k, kb, par, j_rest, n, x.tmp32[0], true));
+      MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
+      have_rest = true;
+    }
+    if (kb1 > 0) {
+      MOPS_TRY(update_cols32(s2, c, x, k, kb, par, j_next, j_next + kb1, x.tmp32[1], false));
+      MOPS_TRY(panel_mixed(s2, c, x, j_next, kb1, par ^ 1));
+      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
+    }
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
+  hipLaunchKernelGGL((k_transpose<float, double>), dim3(ceil_div(n, 64), ceil_div(n, 64)),
+                     dim3(256), 0, s, n, n, x.T32, (long)n, LU, ldlu);
+  MOPS_CHECK_HIP(hipGetLastError());
+  if (ipiv) {
+    hipLaunchKernelGGL(k_piv_to_i64, dim3(grid1(n)), dim3(256), 0, s, c.piv, ipiv, n);
+    MOPS_CHECK_HIP(hipGetLastError());
+  }
+  if (perm_out) MOPS_CHECK_HIP(hipMemcpyAsync(perm_out, c.perm, (size_t)n * sizeof(int),
                                               hipMemcpyDeviceToDevice, s));
   return ST_OK;
 }

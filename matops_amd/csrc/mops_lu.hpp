@@ -19,4 +19,26 @@ size_t getrs_ws_bytes(int n, int nrhs);
 int getrs_d(hipStream_t s, int flags, int n, int nrhs, const double* A, long lda,
             const long long* ipiv, int npiv, const int* perm, double* B, long ldb);
 
+// Mixed precision LU of the n x n matrix A (read only): the trailing matrix
+// in FP32 (FP32 matrix cores), each panel widened to FP64 and factored by the
+// FP64 panel code. LU (n x n, FP64) receives the factors (their FP32
+// values), perm_out the row permutation, ipiv the pivots.
+size_t getrf_mixed_ws_bytes(int n, int nb);
+int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, double* LU, long ldlu,
+                  long long* ipiv, int* perm_out, int* err_flag);
+
+}  // namespace mops
+
+#include "../../include/mops.h"
+
+namespace mops {
+
+// A X = B by LU in precision prec (0 = FP64, 1 = FP32) plus FP64 iterative
+// refinement until the normwise backward error is <= tol (at most maxit
+// solves). X may not alias B.
+size_t gesv_ir_ws_bytes(int n, int nrhs, int prec);
+int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const double* B,
+              long ldb, double* X, long ldx, int prec, double tol, int maxit,
+              mops_ir_report* res);
+
 }  // namespace mops

@@ -202,6 +202,31 @@ def CholeskyAdjoint(L, Lbar, nb=64):
     return Lbar, None
 
 
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+
+def SolveIR(X, A, B, prec=PREC_FP32, tol=1e-12, maxit=10):
+    """X = A^-1 B by LU in precision prec plus FP64 iterative refinement
+    (no counterpart in the reference). A and B are not modified. Returns the
+    report as a dict (iterations, converged, backward_error, first_error,
+    factor_ms, refine_ms)."""
+    n = A.shape[0]
+    if A.shape != (n, n) or B.shape[0] != n or X.shape != B.shape:
+        raise ValueError("SolveIR: size mismatch")
+    rep = _lib.IrReport()
+    if n == 0 or B.shape[1] == 0:
+        return {"iterations": 0, "converged": 1, "backward_error": 0.0, "first_error": 0.0,
+                "factor_ms": 0.0, "refine_ms": 0.0}
+    Am, _, _ = _view(A)
+    Bm, _, _ = _view(B)
+    Xm, _, _ = _view(X)
+    check("mops_dgesv_ir", _lib.lib().mops_dgesv_ir(
+        ctypes.byref(Am), ctypes.byref(Bm), ctypes.byref(Xm), n, B.shape[1], prec, tol, maxit,
+        ctypes.byref(rep)))
+    return {f: getattr(rep, f) for f, _ in rep._fields_}
+
+
 def DecomposeLU(A, pivots, nb):
     """LU with partial pivoting in place (lu.go:257-273). pivots: int64
     numpy array (Go []int), at least min(rows, cols) long. Returns (A, err)."""
