@@ -47,12 +47,31 @@ def parse():
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the DGEMM 1024^3 line")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (DGEMM 1024^3, LU and mixed LU n=16384, "
+                         "Cholesky adjoint n=4096)")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="one warm-up and one timed step plus a 256 MiB reference copy, for "
+                         "rocprofv3 counter passes; prints no JSON")
+    ap.add_argument("--only", choices=["dgemm", "lu", "mixed", "adjoint"],
+                    help="run only this other workload (for profiling) and print its JSON")
+    ap.add_argument("--kernel-stats", metavar="DB",
+                    help="print the per-kernel duration table of a rocprofv3 --kernel-trace "
+                         "database and exit")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="turn two rocprofv3 --pmc databases of a --profile-run (FETCH_SIZE, "
+                         "then WRITE_SIZE) into profiles/traffic.json and exit")
     return ap.parse_args()
 
 
 def main():
     args = parse()
+    if args.traffic_from:
+        return traffic_summary(*args.traffic_from)
+    if args.kernel_stats:
+        return kernel_stats(args.kernel_stats)
+    if args.profile_run:
+        args.steps, args.warmup, args.no_cpu, args.no_extra = 1, 1, True, True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path runs on a single MI355X; see DESIGN.md for how it would shard)",
@@ -77,8 +96,21 @@ def main():
         if rc != 0:
             raise RuntimeError(f"{what}: {rc} {_lib.last_error()}")
 
-    # ---- inputs: A = G G^T / n + I (G = H^T, H ~ N(0,1) seed 3), B seed 4
     gen = torch.Generator(device=dev)
+    if args.only:
+        if args.only == "dgemm":
+            res = extra_dgemm(L, ok, colmajor, gen, stream, sp)
+        elif args.only == "lu":
+            res = extra_lu(L, ok, colmajor, torch, dev, sp)[0]
+        elif args.only == "mixed":
+            _, xref = extra_lu(L, ok, colmajor, torch, dev, sp, reps=0)
+            res = extra_mixed(L, ok, colmajor, torch, dev, sp, xref, reps=0)
+        else:
+            res = extra_adjoint(L, ok, colmajor, torch, dev, sp)
+        print(json.dumps({args.only: res}), flush=True)
+        return 0
+
+    # ---- inputs: A = G G^T / n + I (G = H^T, H ~ N(0,1) seed 3), B seed 4
     gen.manual_seed(3)
     H = colmajor(n, n)
     H.normal_(generator=gen)
@@ -126,6 +158,18 @@ def main():
     ok(L.mops_trace_collect(0, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(fl)), "trace")
     L.mops_trace_enable(0)
 
+    if args.profile_run:
+        # reference transfer of known size, 256 MiB read + 256 MiB written, by
+        # a kernel that appears nowhere else in the run (elementwise negation)
+        src = torch.empty(32 << 20, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print("profile run done", flush=True)
+        return 0
+
     # ---- correctness of the timed results (size-independent properties)
     inf = info.cpu().tolist()
     if any(v != -1 for v in inf):
@@ -163,7 +207,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "SYRK trailing update (k_gemm 128x128 NT, lower triangle)",
+            "kernel": "SYRK trailing update, k_gemm<double, 128, 128, 16, 2, 4, false, true> "
+                      "(lower triangle), launches with 128x128 tiles",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP64_MFMA,
             "unit": "TFLOP/s",
@@ -179,39 +224,218 @@ def main():
         with open(traffic) as f:
             out["roofline"]["traffic"] = json.load(f).get("syrk_update_bytes_per_launch")
 
-    # ---- extra: DGEMM 1024^3 (C = A*B, alpha=1, beta=0)
     if not args.no_extra:
-        m = 1024
-        gen.manual_seed(1)
-        A = colmajor(m, m)
-        A.uniform_(-1, 1, generator=gen)
-        gen.manual_seed(2)
-        Bm = colmajor(m, m)
-        Bm.uniform_(-1, 1, generator=gen)
-        C = colmajor(m, m)
-        for _ in range(5):
-            ok(L.mops_d_dgemm(0, m, m, m, 1.0, A.data_ptr(), m, Bm.data_ptr(), m, 0.0,
-                              C.data_ptr(), m, sp), "dgemm")
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        times = []
-        for _ in range(50):
-            e0.record(stream)
-            ok(L.mops_d_dgemm(0, m, m, m, 1.0, A.data_ptr(), m, Bm.data_ptr(), m, 0.0,
-                              C.data_ptr(), m, sp), "dgemm")
-            e1.record(stream)
-            e1.synchronize()
-            times.append(e0.elapsed_time(e1))
-        times.sort()
-        out["dgemm_1024"] = {
-            "median_us": round(times[len(times) // 2] * 1e3, 2),
-            "min_us": round(times[0] * 1e3, 2),
-            "tflops_median": round(2 * m ** 3 / (times[len(times) // 2] * 1e-3) / 1e12, 3),
-        }
+        del As, Bs
+        torch.cuda.empty_cache()
+        out["dgemm_1024"] = extra_dgemm(L, ok, colmajor, gen, stream, sp)
+        out["lu_16384"], xref = extra_lu(L, ok, colmajor, torch, dev, sp)
+        out["mixed_lu_16384"] = extra_mixed(L, ok, colmajor, torch, dev, sp, xref)
+        out["chol_adjoint_4096"] = extra_adjoint(L, ok, colmajor, torch, dev, sp)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nb)
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def extra_dgemm(L, ok, colmajor, gen, stream, sp):
+    """DGEMM 1024^3, C = A*B, alpha = 1, beta = 0, A and B U[-1,1] (seeds 1, 2):
+    median and minimum of 50 launches."""
+    import torch
+    m = 1024
+    gen.manual_seed(1)
+    A = colmajor(m, m)
+    A.uniform_(-1, 1, generator=gen)
+    gen.manual_seed(2)
+    Bm = colmajor(m, m)
+    Bm.uniform_(-1, 1, generator=gen)
+    C = colmajor(m, m)
+    for _ in range(5):
+        ok(L.mops_d_dgemm(0, m, m, m, 1.0, A.data_ptr(), m, Bm.data_ptr(), m, 0.0,
+                          C.data_ptr(), m, sp), "dgemm")
+    e0, e1 = _events(torch)
+    times = []
+    for _ in range(50):
+        e0.record(stream)
+        ok(L.mops_d_dgemm(0, m, m, m, 1.0, A.data_ptr(), m, Bm.data_ptr(), m, 0.0,
+                          C.data_ptr(), m, sp), "dgemm")
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    med = times[len(times) // 2]
+    return {"median_us": round(med * 1e3, 2), "min_us": round(times[0] * 1e3, 2),
+            "tflops_median": round(2 * m ** 3 / (med * 1e-3) / 1e12, 3),
+            "frac_of_peak": round(2 * m ** 3 / (med * 1e-3) / 1e12 / PEAK_FP64_MFMA, 4)}
+
+
+def _lu_problem(torch, dev, colmajor, n, nrhs):
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    A = colmajor(n, n)
+    A.normal_(generator=g)
+    g.manual_seed(6)
+    B = colmajor(n, nrhs)
+    B.normal_(generator=g)
+    return A, B
+
+
+def _eta(torch, A, X, B):
+    R = B - A @ X
+    an = A.abs().sum(dim=1).max()
+    return (R.abs().amax(dim=0) / (an * X.abs().amax(dim=0) + B.abs().amax(dim=0))).max().item()
+
+
+def extra_lu(L, ok, colmajor, torch, dev, sp, n=16384, nrhs=16, reps=3):
+    """DecomposeLU(A, piv, 256) + SolveLU, n = 16384, 16 RHS, N(0,1) (seeds 5, 6):
+    best of `reps` wall times of the device calls (factor + solve)."""
+    A0, B0 = _lu_problem(torch, dev, colmajor, n, nrhs)
+    A = colmajor(n, n)
+    ipiv = torch.zeros(n, dtype=torch.int64, device=dev)
+    perm = torch.zeros(n, dtype=torch.int32, device=dev)
+    st = torch.zeros(1, dtype=torch.int32, device=dev)
+    best = None
+    for _ in range(reps + 1):
+        A.copy_(A0)
+        X = B0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok(L.mops_d_dgetrf(n, n, 256, A.data_ptr(), n, ipiv.data_ptr(), perm.data_ptr(),
+                           st.data_ptr(), sp), "dgetrf")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ok(L.mops_d_dgetrs(0, n, nrhs, A.data_ptr(), n, ipiv.data_ptr(), n, perm.data_ptr(),
+                           X.data_ptr(), n, sp), "dgetrs")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if best is None or t2 - t0 < best[0] + best[1]:
+            best = (t1 - t0, t2 - t1)
+    if st.item() != 0:
+        raise RuntimeError("dgetrf: grid barrier timed out")
+    eta = _eta(torch, A0, X, B0)
+    flops = 2 * n ** 3 / 3 - n ** 2 / 2 + 5 * n / 6
+    return ({"n": n, "nrhs": nrhs, "factor_ms": round(best[0] * 1e3, 2),
+             "solve_ms": round(best[1] * 1e3, 2),
+             "total_ms": round((best[0] + best[1]) * 1e3, 2),
+             "factor_tflops": round(flops / best[0] / 1e12, 3),
+             "backward_error": eta}, X)
+
+
+def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, n=16384, nrhs=16, reps=2):
+    """The same system by FP32 LU + FP64 refinement to eta <= 1e-12, next to
+    the plain FP64 solve (mops_d_dgesv_ir with precision FP64 and FP32); the
+    forward error is against the FP64 GPU solution (pivots identical to
+    LAPACK's, backward error O(eps))."""
+    from matops_amd import _lib
+    A0, B0 = _lu_problem(torch, dev, colmajor, n, nrhs)
+    res = {}
+    for name, prec in (("fp64", 0), ("fp32_ir", 1)):
+        best = None
+        for _ in range(reps + 1):
+            X = colmajor(n, nrhs)
+            rep = _lib.IrReport()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ok(L.mops_d_dgesv_ir(n, nrhs, A0.data_ptr(), n, B0.data_ptr(), n, X.data_ptr(), n,
+                                 prec, 1e-12, 10, ctypes.byref(rep), sp), "dgesv_ir")
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if best is None or t < best[0]:
+                best = (t, rep.iterations, rep.backward_error, rep.converged, rep.factor_ms,
+                        rep.refine_ms)
+        fwd = ((X - xref).abs().amax() / xref.abs().amax()).item()
+        res[name] = {"wall_ms": round(best[0] * 1e3, 2), "iterations": best[1],
+                     "backward_error": best[2], "converged": bool(best[3]),
+                     "factor_ms": round(best[4], 2), "refine_ms": round(best[5], 2),
+                     "forward_error_vs_fp64": fwd,
+                     "backward_error_check": _eta(torch, A0, X, B0)}
+    res["speedup_vs_fp64"] = round(res["fp64"]["wall_ms"] / res["fp32_ir"]["wall_ms"], 3)
+    return res
+
+
+def extra_adjoint(L, ok, colmajor, torch, dev, sp, n=4096):
+    """Cholesky adjoint n = 4096: A = Q diag(logspace(0, 4)) Q^T, W lower
+    N(0,1); time of mops_d_dpotrf_adjoint (reverse sweep 2n^3/3 flops)."""
+    g = torch.Generator(device="cpu").manual_seed(7)
+    X = torch.randn((n, n), dtype=torch.float64, generator=g).to(dev)
+    Q, _ = torch.linalg.qr(X)
+    A = (Q * torch.logspace(0, 4, n, dtype=torch.float64, device=dev)) @ Q.T
+    A = 0.5 * (A + A.T)
+    W = torch.tril(torch.randn((n, n), dtype=torch.float64, generator=g)).to(dev)
+    Lc = colmajor(n, n)
+    Lc.copy_(A)
+    ok(L.mops_d_dpotrf(LOWER, n, 256, Lc.data_ptr(), n, None, sp), "dpotrf")
+    Lc.copy_(torch.tril(Lc))
+    Ab = colmajor(n, n)
+    best = None
+    for _ in range(4):
+        Ab.copy_(W)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok(L.mops_d_dpotrf_adjoint(n, 64, Lc.data_ptr(), n, Ab.data_ptr(), n, sp), "adjoint")
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    return {"n": n, "ms": round(best * 1e3, 2),
+            "tflops": round(2 * n ** 3 / 3 / best / 1e12, 3),
+            "symmetric": bool(torch.equal(Ab, Ab.T))}
+
+
+def kernel_stats(db):
+    """Per-kernel calls / total / average / min / max duration (us) and share,
+    from the kernels view of a rocprofv3 database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, count(*), sum(end - start), avg(end - start), min(end - start), "
+        "max(end - start) from kernels group by name order by sum(end - start) desc").fetchall()
+    tot = sum(r[2] for r in rows) or 1
+    print(f"{'calls':>6} {'total_us':>12} {'avg_us':>10} {'min_us':>10} {'max_us':>10} {'pct':>6}  kernel")
+    for name, cnt, sm, av, mn, mx in rows:
+        short = name.replace("mops::(anonymous namespace)::", "")
+        if len(short) > 110:
+            short = short[:107] + "..."
+        print(f"{cnt:6d} {sm / 1e3:12.1f} {av / 1e3:10.2f} {mn / 1e3:10.2f} {mx / 1e3:10.2f} "
+              f"{100 * sm / tot:6.2f}  {short}")
+    return 0
+
+
+def traffic_summary(fetch_db, write_db):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the SYRK trailing update
+    and of the reference copy, from two rocprofv3 --pmc databases of a
+    --profile-run; HBM bytes per SYRK launch = counter * (copy bytes / copy
+    counter). Writes profiles/traffic.json."""
+    import sqlite3
+
+    def per_kernel(db, counter):
+        con = sqlite3.connect(db)
+        return con.execute("select kernel_name, value from counters_collection "
+                           "where counter_name = ?", (counter,)).fetchall()
+
+    out = {}
+    for db, counter in ((fetch_db, "FETCH_SIZE"), (write_db, "WRITE_SIZE")):
+        rows = per_kernel(db, counter)
+        syrk = [v for name, v in rows if "k_gemm<double, 128, 128, 16, 2, 4, false, true>" in name]
+        copy = [v for name, v in rows if "neg" in name.lower()]
+        if not syrk or not copy:
+            raise RuntimeError(f"{counter}: kernels not found in {db}")
+        out[counter] = {"syrk_kib_avg": sum(syrk) / len(syrk), "syrk_launches": len(syrk),
+                        "copy_kib_avg": sum(copy) / len(copy), "copy_launches": len(copy)}
+    copy_bytes = 256 << 20
+    fs = out["FETCH_SIZE"]
+    ws = out["WRITE_SIZE"]
+    f_scale = copy_bytes / (fs["copy_kib_avg"] * 1024)
+    w_scale = copy_bytes / (ws["copy_kib_avg"] * 1024)
+    traffic = fs["syrk_kib_avg"] * 1024 * f_scale + ws["syrk_kib_avg"] * 1024 * w_scale
+    res = {"syrk_update_bytes_per_launch": traffic, "fetch_scale": f_scale,
+           "write_scale": w_scale, "counters": out}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res, indent=1))
     return 0
 
 

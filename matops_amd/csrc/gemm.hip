@@ -397,8 +397,8 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   g.sC2 = sC2;
   // Large outputs: 128x128 tiles (16 MFMA blocks per wave). Smaller ones use
   // 64x64 tiles so that the grid still covers the 256 CUs.
-  const long t128 = (long)ceil_div(M, 128) * ceil_div(N, 128) * batch * batch2;
-  if (t128 >= 512) return launch_gemm<T, 128, 128, 16, 2, 4>(s, ta, tb, g, batch, batch2);
+  if (gemm_big_tiles(M, N, (long)batch * batch2))
+    return launch_gemm<T, 128, 128, 16, 2, 4>(s, ta, tb, g, batch, batch2);
   return launch_gemm<T, 64, 64, 32, 2, 2>(s, ta, tb, g, batch, batch2);
 }
 

@@ -19,6 +19,13 @@ enum : int { GM_FULL = 0, GM_LOWER = 1, GM_UPPER = 2, GM_LOWER_T = 3, GM_UPPER_T
 //   TRI_B_LOWER: op(B)[k][j] == 0 for k < j      TRI_B_UPPER: k > j
 enum : int { TRI_NONE = 0, TRI_A_LOWER = 1, TRI_A_UPPER = 2, TRI_B_LOWER = 3, TRI_B_UPPER = 4 };
 
+// Tile configuration rule of gemm_d / gemm_s: 128x128 tiles when the output
+// has at least 512 of them, else 64x64 tiles (so the grid still covers the
+// 256 CUs).
+inline bool gemm_big_tiles(long M, long N, long batch = 1) {
+  return ((M + 127) / 128) * ((N + 127) / 128) * batch >= 512;
+}
+
 // C(MxN) = alpha*op(A)*op(B) + beta*C, column-major. beta == 0 never reads C
 // (NaN/Inf in C are cleared, as dscale_tile does at calgo/vec_scale.c:114).
 int gemm_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
