@@ -74,106 +74,185 @@ __global__ __launch_bounds__(256) void k_trtri_leaf(int bb, int lower, int unit,
 
 // ---- LEFT solve for few right-hand sides: one kernel, blocks chained by flags
 //
-// Workgroup (b, g) solves block row b (64 rows) of op(A) X = B for the
-// column group g (16 columns): acc = B_b - sum_c op(A)_bc X_c over the
-// already solved blocks c (waiting on their flags in solve order), then
-// X_b = op(inv(A_bb)) acc, written over B_b, and flag (b, g) is released.
-// Workgroups are numbered in solve order, so every block a workgroup waits
-// for belongs to a workgroup dispatched before it. The products run on the
-// FP64 matrix cores (v_mfma_f64_16x16x4, one 16x16 accumulator per wave).
-constexpr int SK_BS = 64, SK_CG = 16;
+// Workgroup (b, g), 1024 threads, solves block row b (64 rows) of
+// op(A) X = B for the column group g (64 columns); each of its 16 waves
+// owns one 16x16 tile of the block. With p the block solved just before b
+// (b-1 forward, b+1 backward) and D_b = op(inv(A_bb)):
+//   off the chain:  acc = B_b - sum_{c solved before p} op(A)_bc X_c
+//                   (each X_c as soon as its flag is up), y = D_b acc,
+//                   M_b = D_b op(A)_bp
+//   on the chain:   X_b = y - M_b X_p   (one 64x64x64 product)
+// X_b is written over B_b with agent-scope atomic stores, then flag (b, g)
+// goes up; readers load X with agent-scope atomic loads, so no cache
+// maintenance is needed. Workgroups are numbered in solve order, so every
+// block a workgroup waits for belongs to a workgroup dispatched before it.
+// The products run on the FP64 matrix cores (v_mfma_f64_16x16x4).
+constexpr int SK_BS = 64, SK_CG = 64;
+constexpr int SK_LD = SK_BS + 1;
 constexpr unsigned SK_SPIN_LIMIT = 1u << 24;
 
-__global__ __launch_bounds__(256) void k_trsm_skinny(int N, int nrhs, int eff_lower, int trans,
-                                                     const double* __restrict__ A, long lda,
-                                                     const double* __restrict__ Dinv,
-                                                     double* B, long ldb, int* flags, int* err) {
-  __shared__ double At[SK_BS][SK_BS + 1];  // op(A) block [row][k]
-  __shared__ double Xs[SK_BS][SK_CG + 1];  // solved block / accumulator [k][col]
+__device__ __forceinline__ void sk_wait(const int* flag, int* err) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (++spins > SK_SPIN_LIMIT) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_trsm_skinny(int N, int nrhs, int eff_lower, int trans,
+                                                      const double* __restrict__ A, long lda,
+                                                      const double* __restrict__ Dinv,
+                                                      double* B, long ldb, int* flags, int* err) {
+  __shared__ double As[SK_BS * SK_LD];  // op(A) block [row][k]
+  __shared__ double Xs[SK_BS * SK_LD];  // X block [k][col], or acc
+  __shared__ double Ds[SK_BS * SK_LD];  // D_b [row][k]
+  __shared__ double Ms[SK_BS * SK_LD];  // M_b [row][k]
   const int nblk = (N + SK_BS - 1) / SK_BS;
   const int ngrp = (nrhs + SK_CG - 1) / SK_CG;
   const int step = blockIdx.x / ngrp, g = blockIdx.x % ngrp;
   const int b = eff_lower ? step : nblk - 1 - step;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
+  const int li = lane & 15, lq = lane >> 4;
+  const int ti = w & 3, tj = w >> 2;  // this wave's tile: rows 16ti.., columns 16tj..
   const int r0 = b * SK_BS, c0 = g * SK_CG;
+  const int nb_rows = min(SK_BS, N - r0);
+  const int ncols = min(SK_CG, nrhs - c0);
 
-  // acc = B_b in the MFMA result layout: reg r <-> row 16w + lk + 4r, col lr
-  v4d acc;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = r0 + 16 * w + lk + 4 * r, col = c0 + lr;
-    acc[r] = (row < N && col < nrhs) ? B[row + (long)col * ldb] : 0.0;
-  }
-  // op(A)[i][k] of block (bi, bk), staged in At[i][k]
-  auto load_a = [&](const double* base, long ld, int bi, int bk, int ni, int nk, bool tr) {
-    for (int idx = t; idx < SK_BS * SK_BS; idx += 256) {
+  // op(A) block (b, c) into S[i][k] (rows of block b, columns of block c)
+  auto load_opA = [&](double* S, int c) {
+    const int nk = min(SK_BS, N - c * SK_BS);
+    for (int idx = t; idx < SK_BS * SK_BS; idx += 1024) {
       const int x = idx % SK_BS, y = idx / SK_BS;  // x fastest: contiguous in memory
-      double v = 0.0;
-      if (!tr) {  // element (i = x, k = y) at base[bi*64 + x + (bk*64 + y) * ld]
-        if (x < ni && y < nk) v = base[(long)bi * SK_BS + x + ((long)bk * SK_BS + y) * ld];
-        At[x][y] = v;
-      } else {  // op(A)[i][k] = A[k][i]: element (k = x, i = y)
-        if (y < ni && x < nk) v = base[(long)bk * SK_BS + x + ((long)bi * SK_BS + y) * ld];
-        At[y][x] = v;
+      if (!trans) {  // op(A)[i = x][k = y] = A[r0 + x][c*64 + y]
+        S[x * SK_LD + y] = (x < nb_rows && y < nk)
+                               ? A[(long)r0 + x + ((long)c * SK_BS + y) * lda] : 0.0;
+      } else {       // op(A)[i = y][k = x] = A[c*64 + x][r0 + y]
+        S[y * SK_LD + x] = (y < nb_rows && x < nk)
+                               ? A[(long)c * SK_BS + x + ((long)r0 + y) * lda] : 0.0;
       }
     }
   };
-  const int nb_rows = min(SK_BS, N - r0);
-  for (int it = 0; it < step; ++it) {
-    const int c = eff_lower ? it : nblk - 1 - it;
-    const int nc = min(SK_BS, N - c * SK_BS);
-    load_a(A, lda, b, c, nb_rows, nc, trans != 0);
-    if (t == 0) {
-      unsigned spins = 0;
-      while (__hip_atomic_load(&flags[c * ngrp + g], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
-             0) {
-        if (++spins > SK_SPIN_LIMIT) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-    __syncthreads();
-    for (int idx = t; idx < SK_BS * SK_CG; idx += 256) {
+  // X block c (rows of block c, this group's columns) into Xs[k][col]
+  auto load_x = [&](int c) {
+    const int nk = min(SK_BS, N - c * SK_BS);
+    for (int idx = t; idx < SK_BS * SK_CG; idx += 1024) {
       const int k = idx % SK_BS, col = idx / SK_BS;
-      const int gk = c * SK_BS + k, gc = c0 + col;
-      Xs[k][col] = (k < nc && gc < nrhs)
-                       ? __hip_atomic_load(&B[gk + (long)gc * ldb], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)
-                       : 0.0;
+      Xs[k * SK_LD + col] = (k < nk && col < ncols)
+          ? __hip_atomic_load(&B[(long)c * SK_BS + k + (long)(c0 + col) * ldb], __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT)
+          : 0.0;
     }
-    __syncthreads();
+  };
+  // tile (ti, tj) of S (64 x 64, [row][k]) times Xs ([k][col])
+  auto mm = [&](const double* S) {
     v4d d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s4 = 0; s4 < SK_BS; s4 += 4)
-      d = __builtin_amdgcn_mfma_f64_16x16x4f64(At[16 * w + lr][s4 + lk], Xs[s4 + lk][lr], d, 0,
-                                               0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] -= d[r];
-    __syncthreads();
-  }
-  // X_b = op(Dinv_b) * acc ; Dinv_b is the 64x64 inverse of A_bb (ld 64)
-  const double* D = Dinv + (long)b * SK_BS * SK_BS;
-  load_a(D, SK_BS, 0, 0, nb_rows, nb_rows, trans != 0);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) Xs[16 * w + lk + 4 * r][lr] = acc[r];
-  __syncthreads();
-  v4d x = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int s4 = 0; s4 < SK_BS; s4 += 4)
-    x = __builtin_amdgcn_mfma_f64_16x16x4f64(At[16 * w + lr][s4 + lk], Xs[s4 + lk][lr], x, 0, 0,
-                                             0);
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(S[(16 * ti + li) * SK_LD + s4 + lq],
+                                               Xs[(s4 + lq) * SK_LD + 16 * tj + li], d, 0, 0, 0);
+    return d;
+  };
+
+  // acc = B_b tile in the MFMA result layout: reg r <-> row 16ti + lq + 4r, col 16tj + li
+  v4d acc;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = r0 + 16 * w + lk + 4 * r, col = c0 + lr;
-    if (row < N && col < nrhs)
-      __hip_atomic_store(&B[row + (long)col * ldb], x[r], __ATOMIC_RELAXED,
+    const int row = 16 * ti + lq + 4 * r, col = 16 * tj + li;
+    acc[r] = (row < nb_rows && col < ncols) ? B[r0 + row + (long)(c0 + col) * ldb] : 0.0;
+  }
+  const int pred = eff_lower ? b - 1 : b + 1;  // the block solved just before b
+  const bool has_pred = step > 0;
+  // D_b (the inverse of A_bb, transposed for op = transpose) and, through
+  // Xs, M_b = D_b op(A)_{b,pred}: neither depends on any solution, so both
+  // are ready before the first wait
+  {
+    const double* D = Dinv + (long)b * SK_BS * SK_BS;
+    for (int idx = t; idx < SK_BS * SK_BS; idx += 1024) {
+      const int x = idx % SK_BS, y = idx / SK_BS;
+      const double v = (x < nb_rows && y < nb_rows) ? D[x + (long)y * SK_BS] : 0.0;
+      if (!trans)
+        Ds[x * SK_LD + y] = v;
+      else
+        Ds[y * SK_LD + x] = v;
+    }
+  }
+  if (has_pred) {
+    load_opA(Xs, pred);  // Xs[k][col] = op(A)_bp[k][col] (rows of b, columns of pred)
+    __syncthreads();
+    const v4d m = mm(Ds);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ms[(16 * ti + lq + 4 * r) * SK_LD + 16 * tj + li] = m[r];
+  }
+  // off the chain: every block solved before pred; the next op(A) block is
+  // fetched into registers while the current one is used
+  constexpr int PER = SK_BS * SK_BS / 1024;
+  double pre[PER];
+  auto fetch_opA = [&](int c) {
+    const int nk = min(SK_BS, N - c * SK_BS);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = t + q * 1024;
+      const int x = idx % SK_BS, y = idx / SK_BS;
+      pre[q] = !trans ? ((x < nb_rows && y < nk) ? A[(long)r0 + x + ((long)c * SK_BS + y) * lda] : 0.0)
+                      : ((y < nb_rows && x < nk) ? A[(long)c * SK_BS + x + ((long)r0 + y) * lda] : 0.0);
+    }
+  };
+  auto store_opA = [&]() {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = t + q * 1024;
+      const int x = idx % SK_BS, y = idx / SK_BS;
+      if (!trans)
+        As[x * SK_LD + y] = pre[q];
+      else
+        As[y * SK_LD + x] = pre[q];
+    }
+  };
+  const int noff = step - 1;  // blocks solved before pred
+  if (noff > 0) fetch_opA(eff_lower ? 0 : nblk - 1);
+  for (int it = 0; it < noff; ++it) {
+    const int c = eff_lower ? it : nblk - 1 - it;
+    __syncthreads();  // As and Xs are free
+    store_opA();
+    if (it + 1 < noff) fetch_opA(eff_lower ? it + 1 : nblk - 2 - it);
+    if (t == 0) sk_wait(&flags[c * ngrp + g], err);
+    __syncthreads();
+    load_x(c);
+    __syncthreads();
+    const v4d d = mm(As);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] -= d[r];
+  }
+  // y = D_b acc
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Xs[(16 * ti + lq + 4 * r) * SK_LD + 16 * tj + li] = acc[r];
+  __syncthreads();
+  v4d y = mm(Ds);
+  if (has_pred) {
+    // on the chain: X_b = y - M_b X_pred
+    __syncthreads();
+    if (t == 0) sk_wait(&flags[pred * ngrp + g], err);
+    __syncthreads();
+    load_x(pred);
+    __syncthreads();
+    const v4d z = mm(Ms);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[r] -= z[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = 16 * ti + lq + 4 * r, col = 16 * tj + li;
+    if (row < nb_rows && col < ncols)
+      __hip_atomic_store(&B[r0 + row + (long)(c0 + col) * ldb], y[r], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
   }
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (t == 0) __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0) __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -339,7 +418,7 @@ int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const dou
     MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, SK_BS, A, lda, D));
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nb64 * ngrp + 1, 256)), dim3(256), 0, s, fl,
                        nb64 * ngrp + 1);
-    hipLaunchKernelGGL(k_trsm_skinny, dim3(nb64 * ngrp), dim3(256), 0, s, N, nrhs,
+    hipLaunchKernelGGL(k_trsm_skinny, dim3(nb64 * ngrp), dim3(1024), 0, s, N, nrhs,
                        eff_lower ? 1 : 0, trans ? 1 : 0, A, lda, D, B, ldb, fl,
                        fl + nb64 * ngrp);
     MOPS_CHECK_HIP(hipGetLastError());
