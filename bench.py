@@ -448,5 +448,20 @@ def cpu_baseline(nb):
     return orc.bench_cholesky_sample(nb=nb)
 
 
+def _release():
+    """Free this thread's device workspace and streams before the runtime
+    shuts down."""
+    try:
+        import torch
+        from matops_amd import _lib
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+            _lib.lib().mops_thread_release()
+    except Exception:
+        pass
+
+
 if __name__ == "__main__":
-    sys.exit(main())
+    rc = main()
+    _release()
+    sys.exit(rc)
