@@ -81,9 +81,11 @@ __device__ __forceinline__ double rsqrt_newton(double d) {
 // The MFMA result layout of a solved block X_a^T is exactly the B-operand
 // layout the later blocks need, so solved blocks stay in registers.
 // The UPPER case runs the same on U^T (rows of U^T = columns of U).
-__global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int upper, double* A, long lda,
+template <bool UPPER>
+__global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int pw, double* A, long lda,
                                                     double* B, double* Minv, int goff, int* info,
                                                     double* badval, int* flag, int* err) {
+  constexpr bool upper = UPPER;
   __shared__ double s1[LEAF * CL_LD];
   __shared__ double s2[LEAF * CL_LD];
   __shared__ double dv[LEAF];  // WG0: diagonal before the square root
@@ -293,18 +295,121 @@ __global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int upper, do
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int e = 0; e < 4; ++e) Bs[(R0 + li) * CL_LD + 16 * b + lq + 4 * e] = xt[b][e];
+    for (int e = 0; e < 4; ++e) {
+      const int c = 16 * b + lq + 4 * e;
+      Bs[(R0 + li) * CL_LD + c] = c < n ? xt[b][e] : 0.0;  // zero beyond the leaf
+    }
   __syncthreads();
-  for (int idx = t; idx < LEAF * LEAF; idx += CL_T) {
+  // This workgroup's rows of L are also columns of the rest of the panel
+  // when bi <= q: those are published (agent-scope atomics and a flag) for
+  // the other workgroups' updates.
+  const int bi = blockIdx.x;
+  const int q = (pw + LEAF - 1) / LEAF;  // column blocks of the panel right of the leaf
+  const bool publish = bi <= q;
+#pragma unroll
+  for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+    const int idx = t + q * CL_T;
     const int a0 = idx % LEAF, a1 = idx / LEAF;
     const int r = upper ? a1 : a0, c = upper ? a0 : a1;
     if (row0 + r < mb && c < n) {
       const double v = Bs[r * CL_LD + c];
-      if (upper)
-        B[c + (long)(row0 + r) * lda] = v;
+      double* p = upper ? &B[c + (long)(row0 + r) * lda] : &B[(row0 + r) + (long)c * lda];
+      if (publish)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       else
-        B[(row0 + r) + (long)c * lda] = v;
+        *p = v;
     }
+  }
+  if (pw <= 0) return;
+  if (publish) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&flag[bi], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // The rest of the panel (the SYR steps of chol.go:44 for columns right of
+  // the leaf, inside the panel): for column block cb, rows of this
+  // workgroup, A[r][c] -= sum_k X[r][k] L[c][k], L[c][.] = the X of
+  // workgroup cb + 1; lower triangle only (row >= column).
+  const int ncb = min(bi, q);
+  for (int cb = 0; cb < ncb; ++cb) {
+    const double* Lc = Bs;  // own rows when cb + 1 == bi
+    if (cb + 1 != bi) {
+      if (t == 0) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(&flag[cb + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+          if (++spins > LEAF_SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __syncthreads();
+      double v[LEAF * LEAF / CL_T];  // all loads in flight before the first LDS write
+#pragma unroll
+      for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+        const int idx = t + q * CL_T;
+        const int a0 = idx % LEAF, a1 = idx / LEAF;
+        const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // L row 64 cb + r, column c
+        v[q] = 0.0;
+        if (LEAF * cb + r < mb && c < n)
+          v[q] = upper ? ld_agent(&B[c + (long)(LEAF * cb + r) * lda])
+                       : ld_agent(&B[(LEAF * cb + r) + (long)c * lda]);
+      }
+#pragma unroll
+      for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+        const int idx = t + q * CL_T;
+        const int a0 = idx % LEAF, a1 = idx / LEAF;
+        const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+        Ls[r * CL_LD + c] = v[q];
+      }
+      __syncthreads();
+      Lc = Ls;
+    }
+    // wave w: rows 16w.. of this chunk, all 64 columns of block cb. The
+    // product is formed so that lanes run along the contiguous index of
+    // the trailing block: lower, acc[tj][e] = C^T[16 tj + lq + 4e][R0 + li]
+    // (rows in lanes); upper, acc[tj][e] = C[R0 + lq + 4e][16 tj + li].
+    v4d acc[4];
+    const int colbase = n + LEAF * cb;  // panel column of block cb, relative to the leaf
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = upper ? R0 + lq + 4 * e : R0 + li;
+        const int jj = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
+        double v = 0.0;
+        if (row0 + r < mb && LEAF * cb + jj < pw && row0 + r >= LEAF * cb + jj)
+          v = upper ? B[(colbase + jj) + (long)(row0 + r) * lda]
+                    : B[(row0 + r) + (long)(colbase + jj) * lda];
+        acc[tj][e] = v;
+      }
+#pragma unroll
+    for (int s4 = 0; s4 < LEAF; s4 += 4) {
+      const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const double lb = Lc[(16 * tj + li) * CL_LD + s4 + lq];
+        if constexpr (UPPER)
+          acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, lb, acc[tj], 0, 0, 0);
+        else
+          acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lb, xa, acc[tj], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = upper ? R0 + lq + 4 * e : R0 + li;
+        const int jj = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
+        if (row0 + r < mb && LEAF * cb + jj < pw && row0 + r >= LEAF * cb + jj) {
+          if (upper)
+            B[(colbase + jj) + (long)(row0 + r) * lda] = acc[tj][e];
+          else
+            B[(row0 + r) + (long)(colbase + jj) * lda] = acc[tj][e];
+        }
+      }
+    __syncthreads();  // Ls is reused by the next block
   }
 }
 
@@ -325,32 +430,45 @@ __global__ void k_fix_info(int* info, const int* err) {
   else if (*info == 0x7fffffff) *info = -1;
 }
 
-// Factor the panel of columns [j0, j0+w) (rows j0..N-1) in place: the
-// recursive (left half, trapezoidal update of the right half, right half)
-// scheme of a Cholesky panel, with 64-column leaves (k_chol_leaf: the
-// diagonal block and the rows below it in one launch). The UPPER case is
-// the transpose of each step.
+// Factor the panel of columns [j0, j0+w) (rows j0..N-1) in place. Up to
+// CL_FUSE columns: 64-column leaves from left to right, each one launch of
+// k_chol_leaf (the diagonal block, the rows below it, and the update of the
+// panel's columns right of the leaf). Wider panels: the recursive (left
+// half, trapezoidal update of the right half, right half) scheme. The UPPER
+// case is the transpose of each step.
+constexpr int CL_FUSE = 4 * LEAF;
+
 struct LeafFlags {
-  int* flags;    // one zeroed flag per leaf launch
+  int* flags;    // four zeroed flags per leaf launch
   int next;
+  int cap;
   int* err;
   double* minv;  // inverses of the current leaf's four 16 x 16 diagonal blocks
 };
 
 int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
              int* info, double* badval) {
-  if (w <= LEAF) {
-    double* D = A + j0 + (long)j0 * lda;
-    const int m = N - j0 - w;
-    double* B = upper ? A + j0 + (long)(j0 + w) * lda : A + (j0 + w) + (long)j0 * lda;
-    int* flag = fl.flags + fl.next;
-    fl.next += 1;
-    hipLaunchKernelGGL(k_chol_leaf, dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(CL_T), 0, s, w,
-                       m, upper ? 1 : 0, D, lda, B, fl.minv, j0, info, badval, flag, fl.err);
-    MOPS_CHECK_HIP(hipGetLastError());
+  if (w <= CL_FUSE) {
+    for (int jl = j0; jl < j0 + w; jl += LEAF) {
+      const int wl = min(LEAF, j0 + w - jl);
+      const int m = N - jl - wl;
+      const int pw = j0 + w - (jl + wl);  // panel columns right of the leaf
+      double* D = A + jl + (long)jl * lda;
+      double* B = upper ? A + jl + (long)(jl + wl) * lda : A + (jl + wl) + (long)jl * lda;
+      if (fl.next + 4 > fl.cap) {
+        set_error("potrf: leaf flags exhausted");
+        return ST_NOMEM;
+      }
+      int* flag = fl.flags + fl.next;
+      fl.next += 4;
+      hipLaunchKernelGGL(upper ? k_chol_leaf<true> : k_chol_leaf<false>,
+                         dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(CL_T), 0, s, wl, m, pw, D,
+                         lda, B, fl.minv, jl, info, badval, flag, fl.err);
+      MOPS_CHECK_HIP(hipGetLastError());
+    }
     return ST_OK;
   }
-  int h = ((w / 2 + LEAF - 1) / LEAF) * LEAF;
+  int h = ((w / 2 + CL_FUSE - 1) / CL_FUSE) * CL_FUSE;
   if (h >= w) h = w - LEAF;
   MOPS_TRY(chol_rec(s, upper, N, j0, h, A, lda, fl, info, badval));
   const int rows = N - j0 - h;
@@ -369,9 +487,15 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
 
 }  // namespace
 
-size_t potrf_ws_bytes(int N, int nb) {
+// Leaf launches of one factorization: each panel contributes ceil(kb / 64)
+// leaves, and its recursive split at most one more.
+static int potrf_leaf_count(int N, int nb) {
   if (nb <= 0 || nb > N) nb = N;
-  return (size_t)(ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 8) * sizeof(int) +
+  return ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 2;
+}
+
+size_t potrf_ws_bytes(int N, int nb) {
+  return (size_t)(4 * potrf_leaf_count(N, nb) + 8) * sizeof(int) +
          (size_t)4 * 256 * sizeof(double) + 2048;
 }
 
@@ -417,12 +541,13 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   // unblocked and blocked algorithms compute the same factor; here both are
   // the blocked one with the outer block clamped to N.
   if (nb <= 0 || nb > N) nb = N;
-  // one flag per 64-column leaf, plus an error word
-  const int nleaf = ceil_div(N, LEAF) + 2 * ceil_div(N, nb) + 4;
+  // four flags per 64-column leaf, plus an error word
+  const int nleaf = 4 * potrf_leaf_count(N, nb);
   LeafFlags fl;
   fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
   fl.minv = (double*)ws_carve((size_t)4 * 256 * sizeof(double));
   fl.next = 0;
+  fl.cap = nleaf;
   if (!fl.flags || !fl.minv) {
     set_error("potrf: workspace too small");
     return ST_NOMEM;

@@ -130,7 +130,8 @@ def test_rankupdatesym(flags, n, k):
 
 
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
-@pytest.mark.parametrize("n,nb", [(7, 0), (60, 16), (300, 68), (513, 256), (1024, 256)])
+@pytest.mark.parametrize("n,nb", [(7, 0), (60, 16), (300, 68), (513, 256), (1024, 256), (700, 0),
+                                  (1000, 200), (2500, 256)])
 def test_chol_factor_and_solve(uplo, n, nb):
     A0 = spd(n, 3)
     A = A0.copy(order="F")
