@@ -49,167 +49,104 @@ __device__ __forceinline__ double rsqrt_newton(double d) {
   return fma(0.5 * y, e, y);
 }
 
-// One 64-column leaf of the Cholesky panel in one launch (256 threads per
-// workgroup, 4 waves).
-//
-// Workgroup 0 factors the n x n diagonal block (n <= 64; rows and columns
-// beyond n are padded with the identity). Wave w owns columns 16w..16w+15,
-// kept as four 16 x 16 MFMA accumulator tiles; the block is factored in four
-// 16-column steps:
-//  * wave jb moves its tiles to row layout (lane l = row l, 16 registers =
-//    columns; through LDS) and factors its 16 columns alone. Column j is the
-//    reference's column step (chol.go:29-46): L[j][j] = sqrt(d), the column
-//    below scaled by 1/L[j][j] (InvScale, chol.go:42; built with
-//    -ffast-math, the reference's divisions are a multiply by the
-//    reciprocal, here 1/sqrt(d) directly), then the rank-1 update of the
-//    wave's remaining columns (the SYR of chol.go:44) with the column
-//    broadcast from LDS. The next diagonal element is formed from two
-//    readlanes ahead of the full update, which keeps the serial chain per
-//    column short. The finished columns go to LDS (Lt) and to A;
-//  * after one barrier the waves right of it apply the rank-16 update of
-//    their tiles on the matrix cores (the same SYR steps for those 16
-//    columns, v_mfma_f64_16x16x4).
-// A NaN diagonal (chol.go:33-36) is reported once, for the first column.
-// Wave 0 then inverts the four 16 x 16 diagonal blocks (lane = block and
-// column of the inverse, forward substitution) into Minv, and a flag
-// releases L and the inverses.
-//
-// Workgroups b >= 1 load 64 rows below the block into LDS (B), wait for the
-// flag, and solve X L^T = B (the TRSM of chol.go:79 for this leaf) on the
-// matrix cores, 16 rows per wave, 16 columns at a time:
-//   X_b^T = inv(L_bb) (B_b^T - sum_{a<b} L_ba X_a^T)
-// The MFMA result layout of a solved block X_a^T is exactly the B-operand
-// layout the later blocks need, so solved blocks stay in registers.
-// The UPPER case runs the same on U^T (rows of U^T = columns of U).
+// Element (r, c), r >= c, of the panel's factor L in memory: the panel's
+// corner is P; LOWER stores L itself, UPPER its transpose U = L^T.
 template <bool UPPER>
-__global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int pw, double* A, long lda,
-                                                    double* B, double* Minv, int goff, int* info,
-                                                    double* badval, int* flag, int* err) {
+__device__ __forceinline__ double* lp(double* P, long lda, int r, int c) {
+  return UPPER ? P + c + (long)r * lda : P + r + (long)c * lda;
+}
+
+// Thread 0 waits for a flag set by another workgroup of the same launch
+// (always one with a lower index, so it has been dispatched); a wait that
+// exceeds the spin limit raises err and gives up instead of hanging.
+__device__ __forceinline__ void cp_wait(const int* f, int* err) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (++spins > LEAF_SPIN_LIMIT) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+constexpr int CP_FLAGS = 4 + 16;  // ints per panel launch
+
+// One Cholesky panel of up to 256 columns (the diagonal blocks of 64 and
+// every row below them) in one launch of 256-thread workgroups.
+//
+// Rows are split into 64-row chunks: workgroup b < nsteps owns the rows of
+// diagonal block b, the others 64 rows each below the panel's diagonal.
+// Block column j is the reference's step for 64 columns (chol.go:29-46 on
+// the diagonal block, the TRSM of chol.go:79 below it, the SYR updates of
+// the columns right of it); here step j is:
+//  * workgroup j factors its diagonal block, kept since the start of the
+//    launch as four 16 x 16 MFMA accumulator tiles per wave (wave w owns
+//    columns 16w..16w+15). It moves its 16 columns to row layout (through
+//    LDS) and factors them one column at a time: L[j][j] = sqrt(d), the
+//    column below scaled by 1/L[j][j] (InvScale, chol.go:42; built with
+//    -ffast-math, the reference's divisions are a multiply by the
+//    reciprocal, here 1/sqrt(d) directly), the rank-1 update of the wave's
+//    remaining columns (the SYR of chol.go:44). The next diagonal element is
+//    formed from two readlanes ahead of the full update, which keeps the
+//    serial chain per column short. After one barrier the waves right of it
+//    apply the rank-16 update of their tiles on the matrix cores. A NaN
+//    diagonal (chol.go:33-36) is reported once, for the first column. Wave
+//    0 then inverts the four 16 x 16 diagonal blocks (lane = block and
+//    column, forward substitution), and a flag publishes L and the
+//    inverses;
+//  * every workgroup below it waits for that flag and solves its rows,
+//    X L^T = B, 16 rows per wave, 16 columns at a time:
+//      X_b^T = inv(L_bb) (B_b^T - sum_{a<b} L_ba X_a^T)
+//    (the MFMA result layout of a solved block X_a^T is the B-operand
+//    layout the later blocks need, so solved blocks stay in registers);
+//  * then it subtracts X X_c^T from its block in each column block c > j
+//    of the panel, X_c being the rows just solved by the diagonal
+//    workgroup c (published with agent-scope stores and a flag). The block
+//    of column j + 1 stays in LDS as the next step's right-hand side; a
+//    diagonal workgroup's own diagonal block is updated in its registers.
+// So the critical path of a step is one diagonal factorization, one
+// 64-row solve and one 64 x 64 update, all on chip; every wait is on a
+// workgroup with a lower index. The UPPER case runs the same on U^T.
+template <bool UPPER>
+__global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, long lda,
+                                                     double* Minv, int goff, int* info,
+                                                     double* badval, int* flags, int* err) {
   constexpr bool upper = UPPER;
   __shared__ double s1[LEAF * CL_LD];
   __shared__ double s2[LEAF * CL_LD];
-  __shared__ double dv[LEAF];  // WG0: diagonal before the square root
+  __shared__ double dv[LEAF];  // diagonal before the square root
   const int t = threadIdx.x, l = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int li = l % 16, lq = l / 16;
-  if (blockIdx.x == 0) {
-    double* Lt = s1;         // Lt[c * 64 + r] = L[r][c], filled block by block
-    double* Cs = s2;         // row-layout staging of the owner's tiles, pitch 17
-    double* rv = s2 + 1152;  // rv[j] = 1 / L[j][j]
-    // tile q of wave w: register e of lane l = A[16q + lq + 4e][16w + li]
-    v4d acc[4];
+  const int R0 = 16 * w;
+  const int nsteps = (pw + LEAF - 1) / LEAF;
+  const int b = blockIdx.x;
+  const bool diag = b < nsteps;
+  const int row0 = diag ? LEAF * b : pw + LEAF * (b - nsteps);
+  const int nr = diag ? min(LEAF, pw - row0) : min(LEAF, m - row0);  // rows held
+  const int jend = diag ? b : nsteps;                                 // solves
+  int* dflag = flags;      // dflag[j]: diagonal block j factored, inverses published
+  int* xflag = flags + 4;  // xflag[4 j + c]: rows of block c solved in step j
+  double* Bs = s1;         // Bs[r * CL_LD + k]: right-hand side, then X, of step j
+  double* Ls = s2;         // L below the 16 x 16 diagonal blocks, or a published X
+
+  // diagonal workgroups: the whole symmetric block (rows and columns beyond
+  // nr: identity); tile q of wave w, register e of lane l = A[16q + lq + 4e][16w + li]
+  v4d dacc[4];
+  if (diag) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = 16 * q + lq + 4 * e, c = 16 * w + li;
+        const int r = 16 * q + lq + 4 * e, c = R0 + li;
         double v = r == c ? 1.0 : 0.0;
-        if (r < n && c < n && c <= r) v = upper ? A[c + (long)r * lda] : A[r + (long)c * lda];
-        acc[q][e] = v;
+        if (r < nr && c < nr)
+          v = c <= r ? *lp<UPPER>(P, lda, row0 + r, row0 + c) : *lp<UPPER>(P, lda, row0 + c, row0 + r);
+        dacc[q][e] = v;
       }
-#pragma unroll 1
-    for (int jb = 0; jb < 4; ++jb) {
-      if (w == jb) {
-        // tiles -> row layout a[i] = (row l, column 16jb + i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) Cs[(16 * q + lq + 4 * e) * 17 + li] = acc[q][e];
-        double a[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) a[i] = Cs[l * 17 + i];
-        double dn = readlane_d(a[0], 16 * jb);
-#pragma unroll
-        for (int ji = 0; ji < 16; ++ji) {
-          const int j = 16 * jb + ji;
-          const double d = dn;
-          double ri, r;
-          if (d > 0.0 && d < INFINITY) {
-            ri = rsqrt_newton(d);
-            r = d * ri;
-          } else {
-            r = sqrt(d);
-            ri = 1.0 / r;
-          }
-          const double x = l > j ? a[ji] * ri : 0.0;
-          if (ji < 15) {
-            const double xn = readlane_d(x, j + 1);
-            dn = fma(-xn, xn, readlane_d(a[ji + 1], j + 1));
-          }
-          a[ji] = l == j ? r : x;
-          Lt[j * 64 + l] = a[ji];
-          dv[j] = d;
-          rv[j] = ri;
-#pragma unroll
-          for (int i = ji + 1; i < 16; ++i) a[i] = fma(-x, Lt[j * 64 + 16 * jb + i], a[i]);
-          // keep this column's updates here: sinking them toward their use
-          // would put chains of dependent FMAs on the serial path
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      }
-      __syncthreads();
-      if (w == jb) {
-        // the finished columns back to A (off the serial path)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int c = 16 * jb + i;
-          const double v = Lt[c * 64 + l];
-          if (l < n && c < n && c <= l) {
-            if (upper)
-              A[c + (long)l * lda] = v;
-            else
-              A[l + (long)c * lda] = v;
-          }
-        }
-      }
-      if (w > jb) {
-        // rank-16 update: tile q (rows 16q..) -= L[rows][block jb] L[16w..][block jb]^T
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (q >= w) {
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-              const double* col = Lt + (16 * jb + 4 * s + lq) * 64;
-              acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-col[16 * q + li], col[16 * w + li],
-                                                            acc[q], 0, 0, 0);
-            }
-          }
-        }
-      }
-    }
-    if (w == 0) {
-      // first NaN on the diagonal
-      const unsigned long long bad = __ballot(l < n && isnan(Lt[l * 64 + l]));
-      if (bad && l == 0 && info) {
-        const int j = __ffsll((long long)bad) - 1;
-        const int old = atomicMin(info, goff + j);
-        if ((old < 0 || old > goff + j) && badval) *badval = dv[j];
-      }
-      // inverses of the diagonal blocks: lane l -> block b = l/16, column
-      // c = l%16; z = inv(L_bb) e_c by forward substitution
-      const int b = lq, c = li;
-      double z[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) z[q] = q == c ? 1.0 : 0.0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        z[q] *= rv[16 * b + q];
-#pragma unroll
-        for (int q2 = q + 1; q2 < 16; ++q2)
-          z[q2] = fma(-Lt[(16 * b + q) * 64 + 16 * b + q2], z[q], z[q2]);
-      }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) Minv[b * 256 + q * 16 + c] = z[q];
-    }
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    return;
   }
-  // ---- 64 rows below the diagonal block
-  double* Bs = s1;  // Bs[r * CL_LD + k] = B[row0 + r][k], later X
-  double* Ls = s2;  // Ls[r * CL_LD + c] = L[r][c] below the diagonal blocks
-  const int row0 = (blockIdx.x - 1) * LEAF;
-  {
+  if (jend > 0) {
     double v[LEAF * LEAF / CL_T];
 #pragma unroll
     for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
@@ -217,8 +154,7 @@ __global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int pw, doubl
       const int a0 = idx % LEAF, a1 = idx / LEAF;  // contiguous index first
       const int r = upper ? a1 : a0, c = upper ? a0 : a1;
       v[q] = 0.0;
-      if (row0 + r < mb && c < n)
-        v[q] = upper ? B[c + (long)(row0 + r) * lda] : B[(row0 + r) + (long)c * lda];
+      if (r < nr && c < min(LEAF, pw)) v[q] = *lp<UPPER>(P, lda, row0 + r, c);
     }
 #pragma unroll
     for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
@@ -228,133 +164,31 @@ __global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int pw, doubl
       Bs[r * CL_LD + c] = v[q];
     }
   }
-  if (t == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-      if (++spins > LEAF_SPIN_LIMIT) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+
+#pragma unroll 1
+  for (int j = 0; j < jend; ++j) {
+    const int c0 = LEAF * j, wj = min(LEAF, pw - c0);
+    if (t == 0) {
+      cp_wait(&dflag[j], err);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  // The acquire by thread 0 invalidated this CU's caches; after the barrier
-  // every thread's loads see what workgroup 0 released.
-  __syncthreads();
-  double mv[4][4];  // inverse blocks in A-operand layout: inv(L_bb)[li][4s + lq]
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int s = 0; s < 4; ++s) mv[b][s] = Minv[b * 256 + li * 16 + 4 * s + lq];
-  {
-    // the factor below the diagonal blocks (the diagonal blocks enter
-    // through their inverses)
-    double v[LEAF * LEAF / CL_T];
-#pragma unroll
-    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
-      const int idx = t + q * CL_T;
-      const int a0 = idx % LEAF, a1 = idx / LEAF;
-      const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // L[r][c], contiguous in memory
-      v[q] = 0.0;
-      if (r < n && c < n && (c >> 4) < (r >> 4))
-        v[q] = upper ? A[c + (long)r * lda] : A[r + (long)c * lda];
-    }
-#pragma unroll
-    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
-      const int idx = t + q * CL_T;
-      const int a0 = idx % LEAF, a1 = idx / LEAF;
-      const int r = upper ? a1 : a0, c = upper ? a0 : a1;
-      Ls[r * CL_LD + c] = v[q];
-    }
-  }
-  __syncthreads();
-  // wave w: rows R0 = 16w .. 16w+15 of this chunk. Lane l, register e of
-  // xt[b] holds X[R0 + l%16][16b + l/16 + 4e].
-  const int R0 = 16 * w;
-  v4d xt[4];
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    v4d y;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) y[e] = Bs[(R0 + li) * CL_LD + 16 * b + lq + 4 * e];
-#pragma unroll
-    for (int a = 0; a < b; ++a)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        // Y^T[c][r] -= sum_k L[16b + c][16a + k] X[r][16a + k]
-        const double lv = -Ls[(16 * b + li) * CL_LD + 16 * a + 4 * s + lq];
-        y = __builtin_amdgcn_mfma_f64_16x16x4f64(lv, xt[a][s], y, 0, 0, 0);
-      }
-    v4d x = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(mv[b][s], y[s], x, 0, 0, 0);
-    xt[b] = x;
-  }
-  // X back to LDS (own rows only), then coalesced stores
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int c = 16 * b + lq + 4 * e;
-      Bs[(R0 + li) * CL_LD + c] = c < n ? xt[b][e] : 0.0;  // zero beyond the leaf
-    }
-  __syncthreads();
-  // This workgroup's rows of L are also columns of the rest of the panel
-  // when bi <= q: those are published (agent-scope atomics and a flag) for
-  // the other workgroups' updates.
-  const int bi = blockIdx.x;
-  const int q = (pw + LEAF - 1) / LEAF;  // column blocks of the panel right of the leaf
-  const bool publish = bi <= q;
-#pragma unroll
-  for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
-    const int idx = t + q * CL_T;
-    const int a0 = idx % LEAF, a1 = idx / LEAF;
-    const int r = upper ? a1 : a0, c = upper ? a0 : a1;
-    if (row0 + r < mb && c < n) {
-      const double v = Bs[r * CL_LD + c];
-      double* p = upper ? &B[c + (long)(row0 + r) * lda] : &B[(row0 + r) + (long)c * lda];
-      if (publish)
-        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      else
-        *p = v;
-    }
-  }
-  if (pw <= 0) return;
-  if (publish) {
-    __builtin_amdgcn_s_waitcnt(0);
+    // the acquire by thread 0 invalidated this CU's caches; after the
+    // barrier every thread's loads see what the diagonal workgroup released
     __syncthreads();
-    if (t == 0) __hip_atomic_store(&flag[bi], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  // The rest of the panel (the SYR steps of chol.go:44 for columns right of
-  // the leaf, inside the panel): for column block cb, rows of this
-  // workgroup, A[r][c] -= sum_k X[r][k] L[c][k], L[c][.] = the X of
-  // workgroup cb + 1; lower triangle only (row >= column).
-  const int ncb = min(bi, q);
-  for (int cb = 0; cb < ncb; ++cb) {
-    const double* Lc = Bs;  // own rows when cb + 1 == bi
-    if (cb + 1 != bi) {
-      if (t == 0) {
-        unsigned spins = 0;
-        while (__hip_atomic_load(&flag[cb + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          if (++spins > LEAF_SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-      }
-      __syncthreads();
-      double v[LEAF * LEAF / CL_T];  // all loads in flight before the first LDS write
+    double mv[4][4];  // inverse blocks in A-operand layout: inv(L_bb)[li][4s + lq]
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) mv[bb][s] = Minv[j * 1024 + bb * 256 + li * 16 + 4 * s + lq];
+    {
+      double v[LEAF * LEAF / CL_T];
 #pragma unroll
       for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
         const int idx = t + q * CL_T;
         const int a0 = idx % LEAF, a1 = idx / LEAF;
-        const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // L row 64 cb + r, column c
+        const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // L[r][c] of block j
         v[q] = 0.0;
-        if (LEAF * cb + r < mb && c < n)
-          v[q] = upper ? ld_agent(&B[c + (long)(LEAF * cb + r) * lda])
-                       : ld_agent(&B[(LEAF * cb + r) + (long)c * lda]);
+        if (r < wj && c < wj && (c >> 4) < (r >> 4)) v[q] = *lp<UPPER>(P, lda, c0 + r, c0 + c);
       }
 #pragma unroll
       for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
@@ -363,54 +197,249 @@ __global__ __launch_bounds__(CL_T) void k_chol_leaf(int n, int mb, int pw, doubl
         const int r = upper ? a1 : a0, c = upper ? a0 : a1;
         Ls[r * CL_LD + c] = v[q];
       }
-      __syncthreads();
-      Lc = Ls;
     }
-    // wave w: rows 16w.. of this chunk, all 64 columns of block cb. The
-    // product is formed so that lanes run along the contiguous index of
-    // the trailing block: lower, acc[tj][e] = C^T[16 tj + lq + 4e][R0 + li]
-    // (rows in lanes); upper, acc[tj][e] = C[R0 + lq + 4e][16 tj + li].
-    v4d acc[4];
-    const int colbase = n + LEAF * cb;  // panel column of block cb, relative to the leaf
+    __syncthreads();
+    // solve; lane l, register e of xt[bb] holds X[R0 + l%16][16bb + l/16 + 4e]
+    v4d xt[4];
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+    for (int bb = 0; bb < 4; ++bb) {
+      v4d y;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) y[e] = Bs[(R0 + li) * CL_LD + 16 * bb + lq + 4 * e];
+#pragma unroll
+      for (int a = 0; a < bb; ++a)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          // Y^T[c][r] -= sum_k L[16bb + c][16a + k] X[r][16a + k]
+          const double lv = -Ls[(16 * bb + li) * CL_LD + 16 * a + 4 * s + lq];
+          y = __builtin_amdgcn_mfma_f64_16x16x4f64(lv, xt[a][s], y, 0, 0, 0);
+        }
+      v4d x = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(mv[bb][s], y[s], x, 0, 0, 0);
+      xt[bb] = x;
+    }
+    // X to LDS (own rows; zero beyond the block's width)
+#pragma unroll
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = upper ? R0 + lq + 4 * e : R0 + li;
-        const int jj = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
-        double v = 0.0;
-        if (row0 + r < mb && LEAF * cb + jj < pw && row0 + r >= LEAF * cb + jj)
-          v = upper ? B[(colbase + jj) + (long)(row0 + r) * lda]
-                    : B[(row0 + r) + (long)(colbase + jj) * lda];
-        acc[tj][e] = v;
+        const int c = 16 * bb + lq + 4 * e;
+        Bs[(R0 + li) * CL_LD + c] = c < wj ? xt[bb][e] : 0.0;
       }
+    __syncthreads();
+    // X is final: to memory; a diagonal workgroup's rows are also a column
+    // block of the panel, published for the other workgroups' updates
 #pragma unroll
-    for (int s4 = 0; s4 < LEAF; s4 += 4) {
-      const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        const double lb = Lc[(16 * tj + li) * CL_LD + s4 + lq];
-        if constexpr (UPPER)
-          acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, lb, acc[tj], 0, 0, 0);
+    for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+      const int idx = t + q * CL_T;
+      const int a0 = idx % LEAF, a1 = idx / LEAF;
+      const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+      if (r < nr && c < wj) {
+        const double v = Bs[r * CL_LD + c];
+        double* p = lp<UPPER>(P, lda, row0 + r, c0 + c);
+        if (diag)
+          __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else
-          acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lb, xa, acc[tj], 0, 0, 0);
+          *p = v;
       }
     }
+    if (diag) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (t == 0) __hip_atomic_store(&xflag[4 * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // updates of the column blocks right of j, right to left, so that the
+    // block of column j + 1 (the next right-hand side) comes last
+    const int chi = diag ? b : nsteps - 1;
+#pragma unroll 1
+    for (int cb = chi; cb > j; --cb) {
+      const int cc0 = LEAF * cb, wcb = min(LEAF, pw - cc0);
+      if (diag && cb == b) {
+        // own diagonal block: dacc -= X X^T; with lanes along the rows the
+        // result lands in the layout of dacc (the block is symmetric)
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+        for (int s4 = 0; s4 < LEAF; s4 += 4) {
+          const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = upper ? R0 + lq + 4 * e : R0 + li;
-        const int jj = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
-        if (row0 + r < mb && LEAF * cb + jj < pw && row0 + r >= LEAF * cb + jj) {
-          if (upper)
-            B[(colbase + jj) + (long)(row0 + r) * lda] = acc[tj][e];
-          else
-            B[(row0 + r) + (long)(colbase + jj) * lda] = acc[tj][e];
+          for (int tj = 0; tj < 4; ++tj)
+            dacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[(16 * tj + li) * CL_LD + s4 + lq], xa,
+                                                            dacc[tj], 0, 0, 0);
+        }
+        continue;
+      }
+      if (t == 0) cp_wait(&xflag[4 * j + cb], err);
+      __syncthreads();
+      {
+        double v[LEAF * LEAF / CL_T];  // all loads in flight before the first LDS write
+#pragma unroll
+        for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+          const int idx = t + q * CL_T;
+          const int a0 = idx % LEAF, a1 = idx / LEAF;
+          const int r = upper ? a1 : a0, c = upper ? a0 : a1;  // X_cb[r][c]
+          v[q] = 0.0;
+          if (r < wcb && c < wj) v[q] = ld_agent(lp<UPPER>(P, lda, cc0 + r, c0 + c));
+        }
+#pragma unroll
+        for (int q = 0; q < LEAF * LEAF / CL_T; ++q) {
+          const int idx = t + q * CL_T;
+          const int a0 = idx % LEAF, a1 = idx / LEAF;
+          const int r = upper ? a1 : a0, c = upper ? a0 : a1;
+          Ls[r * CL_LD + c] = v[q];
         }
       }
-    __syncthreads();  // Ls is reused by the next block
+      __syncthreads();
+      // wave w: rows R0.. of this chunk, the 64 columns of block cb, with
+      // lanes along the contiguous index in memory: LOWER acc[tj][e] =
+      // (row R0 + li, column 16tj + lq + 4e), UPPER (row R0 + lq + 4e,
+      // column 16tj + li)
+      v4d acc[4];
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = upper ? R0 + lq + 4 * e : R0 + li;
+          const int c = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
+          acc[tj][e] = r < nr && c < wcb ? *lp<UPPER>(P, lda, row0 + r, cc0 + c) : 0.0;
+        }
+#pragma unroll
+      for (int s4 = 0; s4 < LEAF; s4 += 4) {
+        const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+          const double lb = Ls[(16 * tj + li) * CL_LD + s4 + lq];
+          if constexpr (UPPER)
+            acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, lb, acc[tj], 0, 0, 0);
+          else
+            acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(lb, xa, acc[tj], 0, 0, 0);
+        }
+      }
+      if (cb == j + 1) {
+        __syncthreads();  // every wave is done with X in Bs
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = upper ? R0 + lq + 4 * e : R0 + li;
+            const int c = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
+            Bs[r * CL_LD + c] = acc[tj][e];
+          }
+      } else {
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = upper ? R0 + lq + 4 * e : R0 + li;
+            const int c = upper ? 16 * tj + li : 16 * tj + lq + 4 * e;
+            if (r < nr && c < wcb) *lp<UPPER>(P, lda, row0 + r, cc0 + c) = acc[tj][e];
+          }
+      }
+      __syncthreads();  // Ls (and Bs) are reused
+    }
   }
+  if (!diag) return;
+
+  // ---- factor the diagonal block b (n = nr columns) ----
+  __syncthreads();
+  const int n = nr;
+  double* A = lp<UPPER>(P, lda, row0, row0);
+  double* Mb = Minv + b * 1024;
+  double* Lt = s1;         // Lt[c * 64 + r] = L[r][c], filled block by block
+  double* Cs = s2;         // row-layout staging of the owner's tiles, pitch 17
+  double* rv = s2 + 1152;  // rv[j] = 1 / L[j][j]
+#pragma unroll 1
+  for (int jb = 0; jb < 4; ++jb) {
+    if (w == jb) {
+      // tiles -> row layout a[i] = (row l, column 16jb + i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Cs[(16 * q + lq + 4 * e) * 17 + li] = dacc[q][e];
+      double a[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) a[i] = Cs[l * 17 + i];
+      double dn = readlane_d(a[0], 16 * jb);
+#pragma unroll
+      for (int ji = 0; ji < 16; ++ji) {
+        const int j = 16 * jb + ji;
+        const double d = dn;
+        double ri, r;
+        if (d > 0.0 && d < INFINITY) {
+          ri = rsqrt_newton(d);
+          r = d * ri;
+        } else {
+          r = sqrt(d);
+          ri = 1.0 / r;
+        }
+        const double x = l > j ? a[ji] * ri : 0.0;
+        if (ji < 15) {
+          const double xn = readlane_d(x, j + 1);
+          dn = fma(-xn, xn, readlane_d(a[ji + 1], j + 1));
+        }
+        a[ji] = l == j ? r : x;
+        Lt[j * 64 + l] = a[ji];
+        dv[j] = d;
+        rv[j] = ri;
+#pragma unroll
+        for (int i = ji + 1; i < 16; ++i) a[i] = fma(-x, Lt[j * 64 + 16 * jb + i], a[i]);
+        // keep this column's updates here: sinking them toward their use
+        // would put chains of dependent FMAs on the serial path
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();
+    if (w == jb) {
+      // the finished columns back to A (off the serial path)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int c = 16 * jb + i;
+        const double v = Lt[c * 64 + l];
+        if (l < n && c < n && c <= l) *lp<UPPER>(A, lda, l, c) = v;
+      }
+    }
+    if (w > jb) {
+      // rank-16 update: tile q (rows 16q..) -= L[rows][block jb] L[16w..][block jb]^T
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= w) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const double* col = Lt + (16 * jb + 4 * s + lq) * 64;
+            dacc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-col[16 * q + li], col[R0 + li],
+                                                           dacc[q], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  if (w == 0) {
+    // first NaN on the diagonal
+    const unsigned long long bad = __ballot(l < n && isnan(Lt[l * 64 + l]));
+    if (bad && l == 0 && info) {
+      const int j = __ffsll((long long)bad) - 1;
+      const int old = atomicMin(info, goff + row0 + j);
+      if ((old < 0 || old > goff + row0 + j) && badval) *badval = dv[j];
+    }
+    // inverses of the diagonal blocks: lane l -> block bb = l/16, column
+    // c = l%16; z = inv(L_bb) e_c by forward substitution
+    const int bb = lq, c = li;
+    double z[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) z[q] = q == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      z[q] *= rv[16 * bb + q];
+#pragma unroll
+      for (int q2 = q + 1; q2 < 16; ++q2)
+        z[q2] = fma(-Lt[(16 * bb + q) * 64 + 16 * bb + q2], z[q], z[q2]);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Mb[bb * 256 + q * 16 + c] = z[q];
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(&dflag[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_zero_flags(int* p, int n) {
@@ -431,41 +460,34 @@ __global__ void k_fix_info(int* info, const int* err) {
 }
 
 // Factor the panel of columns [j0, j0+w) (rows j0..N-1) in place. Up to
-// CL_FUSE columns: 64-column leaves from left to right, each one launch of
-// k_chol_leaf (the diagonal block, the rows below it, and the update of the
-// panel's columns right of the leaf). Wider panels: the recursive (left
-// half, trapezoidal update of the right half, right half) scheme. The UPPER
-// case is the transpose of each step.
+// CL_FUSE columns: one launch of k_chol_panel. Wider panels: the recursive
+// (left part, trapezoidal update of the right part, right part) scheme. The
+// UPPER case is the transpose of each step.
 constexpr int CL_FUSE = 4 * LEAF;
 
 struct LeafFlags {
-  int* flags;    // four zeroed flags per leaf launch
+  int* flags;    // CP_FLAGS zeroed flags per panel launch
   int next;
   int cap;
   int* err;
-  double* minv;  // inverses of the current leaf's four 16 x 16 diagonal blocks
+  double* minv;  // inverses of the current panel's 16 x 16 diagonal blocks (4 x 1024)
 };
 
 int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
              int* info, double* badval) {
   if (w <= CL_FUSE) {
-    for (int jl = j0; jl < j0 + w; jl += LEAF) {
-      const int wl = min(LEAF, j0 + w - jl);
-      const int m = N - jl - wl;
-      const int pw = j0 + w - (jl + wl);  // panel columns right of the leaf
-      double* D = A + jl + (long)jl * lda;
-      double* B = upper ? A + jl + (long)(jl + wl) * lda : A + (jl + wl) + (long)jl * lda;
-      if (fl.next + 4 > fl.cap) {
-        set_error("potrf: leaf flags exhausted");
-        return ST_NOMEM;
-      }
-      int* flag = fl.flags + fl.next;
-      fl.next += 4;
-      hipLaunchKernelGGL(upper ? k_chol_leaf<true> : k_chol_leaf<false>,
-                         dim3(1 + ceil_div(m > 0 ? m : 0, LEAF)), dim3(CL_T), 0, s, wl, m, pw, D,
-                         lda, B, fl.minv, jl, info, badval, flag, fl.err);
-      MOPS_CHECK_HIP(hipGetLastError());
+    const int m = N - j0;  // rows of the panel
+    const int nsteps = ceil_div(w, LEAF);
+    if (fl.next + CP_FLAGS > fl.cap) {
+      set_error("potrf: panel flags exhausted");
+      return ST_NOMEM;
     }
+    int* flag = fl.flags + fl.next;
+    fl.next += CP_FLAGS;
+    hipLaunchKernelGGL(upper ? k_chol_panel<true> : k_chol_panel<false>,
+                       dim3(nsteps + ceil_div(m - w, LEAF)), dim3(CL_T), 0, s, m, w,
+                       A + j0 + (long)j0 * lda, lda, fl.minv, j0, info, badval, flag, fl.err);
+    MOPS_CHECK_HIP(hipGetLastError());
     return ST_OK;
   }
   int h = ((w / 2 + CL_FUSE - 1) / CL_FUSE) * CL_FUSE;
@@ -487,16 +509,16 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
 
 }  // namespace
 
-// Leaf launches of one factorization: each panel contributes ceil(kb / 64)
-// leaves, and its recursive split at most one more.
-static int potrf_leaf_count(int N, int nb) {
+// Panel launches of one factorization: each block column of nb contributes
+// ceil(kb / 256) launches, and its recursive split at most one more.
+static int potrf_panel_count(int N, int nb) {
   if (nb <= 0 || nb > N) nb = N;
-  return ceil_div(N, LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 2;
+  return ceil_div(N, 4 * LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 2;
 }
 
 size_t potrf_ws_bytes(int N, int nb) {
-  return (size_t)(4 * potrf_leaf_count(N, nb) + 8) * sizeof(int) +
-         (size_t)4 * 256 * sizeof(double) + 2048;
+  return (size_t)(CP_FLAGS * potrf_panel_count(N, nb) + 8) * sizeof(int) +
+         (size_t)4 * 1024 * sizeof(double) + 2048;
 }
 
 namespace {
@@ -541,11 +563,11 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   // unblocked and blocked algorithms compute the same factor; here both are
   // the blocked one with the outer block clamped to N.
   if (nb <= 0 || nb > N) nb = N;
-  // four flags per 64-column leaf, plus an error word
-  const int nleaf = 4 * potrf_leaf_count(N, nb);
+  // the flags of every panel launch, plus an error word
+  const int nleaf = CP_FLAGS * potrf_panel_count(N, nb);
   LeafFlags fl;
   fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
-  fl.minv = (double*)ws_carve((size_t)4 * 256 * sizeof(double));
+  fl.minv = (double*)ws_carve((size_t)4 * 1024 * sizeof(double));
   fl.next = 0;
   fl.cap = nleaf;
   if (!fl.flags || !fl.minv) {

@@ -251,27 +251,42 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
   }
 
   // ---- epilogue: lane holds C[m0+..+lr][n0+..+drow(lk, r)] ----
+  // Processed EPI row blocks at a time: all loads of C for the chunk are
+  // issued before its first store, so their latencies overlap (a load
+  // after a store to the same array would otherwise wait for it).
   const T alpha = g.alpha, beta = g.beta;
   T* __restrict__ C = g.C + blockIdx.y * g.sC + blockIdx.z * g.sC2;
   const long ldc = g.ldc;
+  const bool lower = g.mode == GM_LOWER || g.mode == GM_LOWER_T;
+  const bool upper = g.mode == GM_UPPER || g.mode == GM_UPPER_T;
+  constexpr int EPI = FM >= 2 ? 2 : 1;
 #pragma unroll
-  for (int i = 0; i < FM; ++i) {
-    const int gm = m0 + wm * WM + i * 16 + lr;
+  for (int i0 = 0; i0 < FM; i0 += EPI) {
+    bool ok[EPI][FN][4];
+    T cv[EPI][FN][4];
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
+    for (int ii = 0; ii < EPI; ++ii) {
+      const int gm = m0 + wm * WM + (i0 + ii) * 16 + lr;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gn = n0 + wn * WN + j * 16 + Mfma<T>::drow(lk, r);
-        bool ok = gm < M && gn < N;
-        if (g.mode == GM_LOWER || g.mode == GM_LOWER_T) ok = ok && gm >= gn;
-        if (g.mode == GM_UPPER || g.mode == GM_UPPER_T) ok = ok && gm <= gn;
-        if (ok) {
-          T* p = C + gm + (long)gn * ldc;
-          T v = alpha * acc[i][j][r];
-          if (beta != T(0)) v += beta * *p;
-          *p = v;
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gn = n0 + wn * WN + j * 16 + Mfma<T>::drow(lk, r);
+          ok[ii][j][r] = gm < M && gn < N && !(lower && gm < gn) && !(upper && gm > gn);
+          cv[ii][j][r] = T(0);
+          if (beta != T(0) && ok[ii][j][r]) cv[ii][j][r] = C[gm + (long)gn * ldc];
         }
-      }
+    }
+#pragma unroll
+    for (int ii = 0; ii < EPI; ++ii) {
+      const int gm = m0 + wm * WM + (i0 + ii) * 16 + lr;
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gn = n0 + wn * WN + j * 16 + Mfma<T>::drow(lk, r);
+          if (ok[ii][j][r]) C[gm + (long)gn * ldc] = alpha * acc[i0 + ii][j][r] + beta * cv[ii][j][r];
+        }
     }
   }
 }
