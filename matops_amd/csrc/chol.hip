@@ -7,11 +7,12 @@
 //   A21 = A21 * tril(A11)^-T          (SolveTrm RIGHT|LOWER|TRANSA, :79)
 //   A22 = A22 - A21 * A21^T           (RankUpdateSym LOWER, :81)
 // and the UPPER variant with A12 / U11^-T (:82-87). The same step runs here
-// with the panel (diagonal block and the rows below it) factored
-// recursively: 64-column leaves factor their diagonal block in registers
-// (the reference's column algorithm), which also yields the leaf's inverse,
-// and solve the rows below with one GEMM; the halves are joined by a
-// trapezoidal GEMM update. The trailing update of step k runs on the
+// with the panel (diagonal block and the rows below it, up to 256 columns)
+// factored in one launch of k_chol_panel: 64-column diagonal blocks in
+// registers (the reference's column algorithm), their rows below solved and
+// the rest of the panel updated on the matrix cores, the workgroups chained
+// by flags. Wider panels are split recursively with trapezoidal GEMM
+// updates between the parts. The trailing update of step k runs on the
 // caller's stream while the next panel is factored on a second stream.
 #include <climits>
 #include <cmath>
