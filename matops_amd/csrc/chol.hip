@@ -587,26 +587,28 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       MOPS_TRY(chol_panel(s, upper, N, k, kb, A, lda, fl, info, badval));
       const int m = N - k - kb;
       if (m <= 0) break;
-      const bool traced = gemm_big_tiles(m, m);
-      if (traced) trace_mark_begin(s, TR_SYRK_UPDATE);
+      trace_mark_begin(s, TR_SYRK_UPDATE);
       MOPS_TRY(chol_update(s, upper, N, k, kb, 0, m, false, A, lda));
-      if (traced) trace_mark_end(s, TR_SYRK_UPDATE, (double)m * (m + 1) * kb);
+      trace_mark_end(s, TR_SYRK_UPDATE, (double)m * (m + 1) * kb);
     }
   } else {
     // Look-ahead: the panel of step k+1 (and the update of its block column
     // by step k) runs on a second stream while the caller's stream applies
-    // step k to the rest of the trailing matrix.
+    // step k to the rest of the trailing matrix. The rest update starts only
+    // when the block column update is done, at the moment the next panel
+    // becomes ready too: both kernels then enter the dispatcher together,
+    // and the panel's workgroups are not left waiting for CU slots behind
+    // thousands of queued tiles.
     hipStream_t s2 = aux_stream();
-    hipEvent_t e_fork = sync_event(0), e_pan = sync_event(1), e_rest = sync_event(2),
+    hipEvent_t e_fork = sync_event(0), e_col = sync_event(1), e_rest = sync_event(2),
                e_join = sync_event(3);
-    if (!s2 || !e_fork || !e_pan || !e_rest || !e_join) {
+    if (!s2 || !e_fork || !e_col || !e_rest || !e_join) {
       set_error("potrf: could not create the look-ahead stream");
       return ST_HIP;
     }
     MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
     MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
     MOPS_TRY(chol_panel(s2, upper, N, 0, min(nb, N), A, lda, fl, info, badval));
-    MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     bool have_rest = false;
     for (int k = 0; k < N; k += nb) {
       const int kb = min(nb, N - k);
@@ -615,23 +617,19 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       const int kb1 = min(nb, m);  // width of the next block column
       // s2 may touch block column k+1 only after the previous rest update
       if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
-      // caller's stream: rest of the trailing matrix, after panel k
-      MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
+      MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
+      MOPS_CHECK_HIP(hipEventRecord(e_col, s2));  // after panel k and this update
       if (m > kb1) {
         const int mr = m - kb1;
-        // timed (when tracing) for the launches that use the 128x128 tiles,
-        // the kernel the measurement reports on
-        const bool traced = gemm_big_tiles(mr, mr);
-        if (traced) trace_mark_begin(s, TR_SYRK_UPDATE);
+        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
+        // timed when tracing: every launch of the triangle-only GEMM
+        trace_mark_begin(s, TR_SYRK_UPDATE);
         MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, mr, false, A, lda));
-        if (traced) trace_mark_end(s, TR_SYRK_UPDATE, (double)mr * (mr + 1) * kb);
+        trace_mark_end(s, TR_SYRK_UPDATE, (double)mr * (mr + 1) * kb);
         MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
         have_rest = true;
       }
-      // second stream: next block column, then the next panel
-      MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
       MOPS_TRY(chol_panel(s2, upper, N, k + kb, kb1, A, lda, fl, info, badval));
-      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     }
     MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
     MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));

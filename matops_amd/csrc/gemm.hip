@@ -77,18 +77,21 @@ __device__ __forceinline__ int xcd_linear(int bid, int nblk) {
   return x * q + (x < r ? x : r) + bid / NXCD;
 }
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool TA, bool TB>
+// TRI: instance for the triangle-only modes (GM_LOWER / GM_UPPER), which
+// keeps the symmetric rank-k updates apart from the other GEMMs in profiles.
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool TA, bool TB, bool TRI>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
   constexpr int NT = 64 * WGM * WGN;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
-  // Large tiles: row pitch = 128 bytes modulo 256, so the two k-rows read by
-  // lanes 0-31 of one ds_read_b64 fall in disjoint bank halves. Small
-  // tiles keep a short pad so that two workgroups fit one CU's LDS.
-  constexpr int PAD = BM >= 128 ? 16 : 4;
+  // Row pitch = 128 bytes modulo 256 (doubles), so the two k-rows read by
+  // lanes 0-31 of one ds_read_b64 fall in disjoint bank halves.
+  constexpr int PAD = 16;
   constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
-  constexpr int A_PER = BM * BK / NT, B_PER = BN * BK / NT;
-  static_assert(A_PER * NT == BM * BK && B_PER * NT == BN * BK, "tile");
+  // operands move in pairs of consecutive elements along their contiguous
+  // dimension (m or n when not transposed for A / transposed for B, else k)
+  constexpr int A_PP = BM * BK / 2 / NT, B_PP = BN * BK / 2 / NT;
+  static_assert(A_PP * NT * 2 == BM * BK && B_PP * NT * 2 == BN * BK, "tile");
   typedef typename Mfma<T>::acc_t acc_t;
 
   __shared__ T As[2][BK * LDS_A];
@@ -153,72 +156,81 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
 
-  T ra[A_PER], rb[B_PER];
+  typedef T pair_t __attribute__((ext_vector_type(2), aligned(sizeof(T))));
+  pair_t ra[A_PP], rb[B_PP];
   const T* __restrict__ A = g.A + blockIdx.y * g.sA + blockIdx.z * g.sA2;
   const T* __restrict__ B = g.B + blockIdx.y * g.sB + blockIdx.z * g.sB2;
   const int M = g.M, N = g.N, K = g.K;
   const long lda = g.lda, ldb = g.ldb;
+  // tiles inside M x N take slabs inside K without bounds checks
+  const bool inner_mn = m0 + BM <= M && n0 + BN <= N;
 
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int e = 0; e < A_PER; ++e) {
-      const int idx = tid + e * NT;
-      int i, kk;
-      if (!TA) {
-        i = idx % BM;
-        kk = idx / BM;
-      } else {
-        kk = idx % BK;
-        i = idx / BK;
+  // element e of this thread's share of a slab: pair start (i, kk) in the
+  // operand's (m or n, k) coordinates; the pair runs along i when the
+  // operand is contiguous in i (CI), else along kk
+  auto coords = [&](int e, int rows, bool ci, int& i, int& kk) {
+    const int idx = tid + e * NT;
+    if (ci) {
+      i = 2 * (idx % (rows / 2));
+      kk = idx / (rows / 2);
+    } else {
+      kk = 2 * (idx % (BK / 2));
+      i = idx / (BK / 2);
+    }
+  };
+  auto fetch = [&](const T* X, long ldx, bool ci, int i0, int lim, int i, int k, bool inner) {
+    const T* p = ci ? X + (i0 + i) + (long)k * ldx : X + k + (long)(i0 + i) * ldx;
+    if (inner) return *(const pair_t*)p;
+    pair_t v = {T(0), T(0)};
+    const long step = ci ? 1 : ldx;
+    if (ci) {
+      if (k < K) {
+        if (i0 + i < lim) v.x = p[0];
+        if (i0 + i + 1 < lim) v.y = p[step];
       }
-      const int gi = m0 + i, gk = k0 + kk;
-      T v = T(0);
-      if (gi < M && gk < K) v = TA ? A[gk + (long)gi * lda] : A[gi + (long)gk * lda];
-      ra[e] = v;
+    } else if (i0 + i < lim) {
+      if (k < K) v.x = p[0];
+      if (k + 1 < K) v.y = p[1];
+    }
+    return v;
+  };
+  auto load = [&](int k0) {
+    const bool inner = inner_mn && k0 + BK <= K;
+#pragma unroll
+    for (int e = 0; e < A_PP; ++e) {
+      int i, kk;
+      coords(e, BM, !TA, i, kk);
+      ra[e] = fetch(A, lda, !TA, m0, M, i, k0 + kk, inner);
     }
 #pragma unroll
-    for (int e = 0; e < B_PER; ++e) {
-      const int idx = tid + e * NT;
+    for (int e = 0; e < B_PP; ++e) {
       int j, kk;
-      if (TB) {
-        j = idx % BN;
-        kk = idx / BN;
-      } else {
-        kk = idx % BK;
-        j = idx / BK;
-      }
-      const int gj = n0 + j, gk = k0 + kk;
-      T v = T(0);
-      if (gj < N && gk < K) v = TB ? B[gj + (long)gk * ldb] : B[gk + (long)gj * ldb];
-      rb[e] = v;
+      coords(e, BN, TB, j, kk);
+      rb[e] = fetch(B, ldb, TB, n0, N, j, k0 + kk, inner);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < A_PER; ++e) {
-      const int idx = tid + e * NT;
+    for (int e = 0; e < A_PP; ++e) {
       int i, kk;
+      coords(e, BM, !TA, i, kk);
       if (!TA) {
-        i = idx % BM;
-        kk = idx / BM;
+        *(pair_t*)&As[buf][kk * LDS_A + i] = ra[e];
       } else {
-        kk = idx % BK;
-        i = idx / BK;
+        As[buf][kk * LDS_A + i] = ra[e].x;
+        As[buf][(kk + 1) * LDS_A + i] = ra[e].y;
       }
-      As[buf][kk * LDS_A + i] = ra[e];
     }
 #pragma unroll
-    for (int e = 0; e < B_PER; ++e) {
-      const int idx = tid + e * NT;
+    for (int e = 0; e < B_PP; ++e) {
       int j, kk;
+      coords(e, BN, TB, j, kk);
       if (TB) {
-        j = idx % BN;
-        kk = idx / BN;
+        *(pair_t*)&Bs[buf][kk * LDS_B + j] = rb[e];
       } else {
-        kk = idx % BK;
-        j = idx / BK;
+        Bs[buf][kk * LDS_B + j] = rb[e].x;
+        Bs[buf][(kk + 1) * LDS_B + j] = rb[e].y;
       }
-      Bs[buf][kk * LDS_B + j] = rb[e];
     }
   };
 
@@ -336,14 +348,22 @@ int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int b
     return ST_BADARG;
   }
   dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)batch2), block(64 * WGM * WGN);
-  if (!ta && !tb)
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, false, false>), grid, block, 0, s, g);
-  else if (ta && !tb)
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, true, false>), grid, block, 0, s, g);
-  else if (!ta && tb)
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, false, true>), grid, block, 0, s, g);
-  else
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, true, true>), grid, block, 0, s, g);
+  const bool tri = g.mode == GM_LOWER || g.mode == GM_UPPER;
+#define MOPS_GEMM_LAUNCH(TA_, TB_)                                                              \
+  if (tri)                                                                                     \
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, TA_, TB_, true>), grid, block, 0, s, g); \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, TA_, TB_, false>), grid, block, 0, s, g);
+  if (!ta && !tb) {
+    MOPS_GEMM_LAUNCH(false, false)
+  } else if (ta && !tb) {
+    MOPS_GEMM_LAUNCH(true, false)
+  } else if (!ta && tb) {
+    MOPS_GEMM_LAUNCH(false, true)
+  } else {
+    MOPS_GEMM_LAUNCH(true, true)
+  }
+#undef MOPS_GEMM_LAUNCH
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }
@@ -410,11 +430,12 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   g.sA2 = sA2;
   g.sB2 = sB2;
   g.sC2 = sC2;
-  // Large outputs: 128x128 tiles (16 MFMA blocks per wave). Smaller ones use
-  // 64x64 tiles so that the grid still covers the 256 CUs.
-  if (gemm_big_tiles(M, N, (long)batch * batch2))
+  // Large outputs: 128x128 tiles (8 MFMA blocks per wave, 8 waves). Smaller
+  // ones, and the triangle-only updates (whose diagonal tiles do wasted
+  // work and whose grids balance better in finer pieces), use 64x64 tiles.
+  if (mode != GM_LOWER && mode != GM_UPPER && gemm_big_tiles(M, N, (long)batch * batch2))
     return launch_gemm<T, 128, 128, 16, 2, 4>(s, ta, tb, g, batch, batch2);
-  return launch_gemm<T, 64, 64, 32, 2, 2>(s, ta, tb, g, batch, batch2);
+  return launch_gemm<T, 64, 64, 16, 2, 2>(s, ta, tb, g, batch, batch2);
 }
 
 }  // namespace
