@@ -231,9 +231,25 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
     if (wave == 1 && k >= rg0 && k < rg0 + LEAF_R && lane <= W) st_agent(&rkj[lane], okl[lane]);
     target += (unsigned)G;
     grid_sync(a.ctr, target, a.err);
-    // 3. every candidate and the old row k, in one round trip
-    for (int idx = t; idx < G * SW; idx += LEAF_R) xs[idx] = ld_agent(&xbj[idx]);
-    if (t >= LEAF_R - (W + 1)) xs[G * SW + t - (LEAF_R - (W + 1))] = ld_agent(&rkj[t - (LEAF_R - (W + 1))]);
+    // 3. every candidate and the old row k, in one round trip (all loads
+    // issued before the first LDS write)
+    {
+      constexpr int XPT = (LEAF_GMAX * SW + LEAF_R - 1) / LEAF_R;
+      double xv[XPT];
+#pragma unroll
+      for (int q = 0; q < XPT; ++q) {
+        const int idx = t + q * LEAF_R;
+        xv[q] = idx < G * SW ? ld_agent(&xbj[idx]) : 0.0;
+      }
+      const bool rk_lane = t >= LEAF_R - (W + 1);
+      const double rk = rk_lane ? ld_agent(&rkj[t - (LEAF_R - (W + 1))]) : 0.0;
+#pragma unroll
+      for (int q = 0; q < XPT; ++q) {
+        const int idx = t + q * LEAF_R;
+        if (idx < G * SW) xs[idx] = xv[q];
+      }
+      if (rk_lane) xs[G * SW + t - (LEAF_R - (W + 1))] = rk;
+    }
     __syncthreads();
     // 4. the pivot (every wave agrees on its own): the first workgroup
     // holding the largest key (workgroups are in row order)
@@ -607,6 +623,35 @@ int update_cols(hipStream_t s, LuCtx& c, int k, int kb, int par, int j0, int j1,
 
 }  // namespace
 
+namespace {
+
+// Moves the trailing-update work of a factorization from s to the bulk
+// stream (sr), after everything already queued on s; join_bulk makes s
+// wait for it again. Without a bulk stream sr stays s.
+int fork_bulk(hipStream_t s, hipStream_t& sr) {
+  hipStream_t b = bulk_stream();
+  hipEvent_t e = sync_event(4);
+  if (!b || !e) return ST_OK;
+  MOPS_CHECK_HIP(hipEventRecord(e, s));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(b, e, 0));
+  sr = b;
+  return ST_OK;
+}
+
+int join_bulk(hipStream_t s, hipStream_t sr) {
+  if (sr == s) return ST_OK;
+  hipEvent_t e = sync_event(5);
+  if (!e) {
+    set_error("getrf: no event to join the bulk stream");
+    return ST_HIP;
+  }
+  MOPS_CHECK_HIP(hipEventRecord(e, sr));
+  MOPS_CHECK_HIP(hipStreamWaitEvent(s, e, 0));
+  return ST_OK;
+}
+
+}  // namespace
+
 size_t getrf_ws_bytes(int m, int n, int nb) {
   if (m <= 0 || n <= 0) return 0;
   const int mn = m < n ? m : n;
@@ -684,6 +729,11 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
     set_error("getrf: could not create the look-ahead stream");
     return ST_HIP;
   }
+  // The rest updates go to the bulk stream, which leaves a few CUs to the
+  // panel chain (its kernels are small and latency-bound: without free CUs
+  // each of them waits for GEMM tiles to drain before it starts).
+  hipStream_t sr = s;
+  MOPS_TRY(fork_bulk(s, sr));
   MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
   MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
   MOPS_TRY(panel(s2, c, 0, min(nb, mn), 0));
@@ -695,11 +745,11 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
     const int kb1 = min(nb, mn - k - kb);  // width of the next panel (0 at the end)
     const int j_next = k + kb, j_rest = k + kb + kb1;
     if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
-    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
-    MOPS_TRY(apply_rows(s, c, par, 0, k));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(sr, e_pan, 0));
+    MOPS_TRY(apply_rows(sr, c, par, 0, k));
     if (j_rest < n) {
-      MOPS_TRY(update_cols(s, c, k, kb, par, j_rest, n, c.tmp[0], true));
-      MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
+      MOPS_TRY(update_cols(sr, c, k, kb, par, j_rest, n, c.tmp[0], true));
+      MOPS_CHECK_HIP(hipEventRecord(e_rest, sr));
       have_rest = true;
     }
     if (kb1 > 0) {
@@ -710,6 +760,7 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   }
   MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
   MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
+  MOPS_TRY(join_bulk(s, sr));
 
   // A = T^T
   hipLaunchKernelGGL((k_transpose<double, double>), dim3(ceil_div(n, 64), ceil_div(m, 64)),
@@ -863,6 +914,8 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
     set_error("getrf_mixed: could not create the look-ahead stream");
     return ST_HIP;
   }
+  hipStream_t sr = s;
+  MOPS_TRY(fork_bulk(s, sr));
   MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
   MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
   MOPS_TRY(panel_mixed(s2, c, x, 0, nb, 0));
@@ -874,11 +927,11 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
     const int kb1 = min(nb, n - k - kb);
     const int j_next = k + kb, j_rest = k + kb + kb1;
     if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
-    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
-    MOPS_TRY(apply_rows32(s, c, x, par, 0, k));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(sr, e_pan, 0));
+    MOPS_TRY(apply_rows32(sr, c, x, par, 0, k));
     if (j_rest < n) {
-      MOPS_TRY(update_cols32(s, c, x, k, kb, par, j_rest, n, x.tmp32[0], true));
-      MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
+      MOPS_TRY(update_cols32(sr, c, x, k, kb, par, j_rest, n, x.tmp32[0], true));
+      MOPS_CHECK_HIP(hipEventRecord(e_rest, sr));
       have_rest = true;
     }
     if (kb1 > 0) {
@@ -889,6 +942,7 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   }
   MOPS_CHECK_HIP(hipEventRecord(e_join, s2));
   MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_join, 0));
+  MOPS_TRY(join_bulk(s, sr));
   hipLaunchKernelGGL((k_transpose<float, double>), dim3(ceil_div(n, 64), ceil_div(n, 64)),
                      dim3(256), 0, s, n, n, x.T32, (long)n, LU, ldlu);
   MOPS_CHECK_HIP(hipGetLastError());

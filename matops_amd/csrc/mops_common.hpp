@@ -83,10 +83,16 @@ void thread_release();
 
 // Per-thread stream used by the host-pointer entry points.
 hipStream_t host_stream();
-// Per-thread second stream (default priority) for look-ahead work, and four
-// reusable synchronisation events (index 0..3) of the calling thread.
+// Per-thread second stream (default priority) for look-ahead work, and six
+// reusable synchronisation events (index 0..5) of the calling thread.
 hipStream_t aux_stream();
 hipEvent_t sync_event(int i);
+// Per-thread stream (default priority) whose kernels stay off the first
+// BULK_RESERVED_CUS compute units, for bulk updates that run beside a
+// latency-bound chain of small kernels on another stream: the chain always
+// finds free CUs. nullptr when the device cannot make one.
+constexpr int BULK_RESERVED_CUS = 32;
+hipStream_t bulk_stream();
 
 // Optional timing of selected launches with HIP events recorded on the
 // launching stream (per host thread). Classes name what is timed.

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "mops_common.hpp"
 
 namespace mops {
@@ -19,7 +21,9 @@ struct ThreadState {
   Workspace ws[16];
   hipStream_t stream[16] = {nullptr};
   hipStream_t aux[16] = {nullptr};
-  hipEvent_t ev[16][4] = {};
+  hipStream_t bulk[16] = {nullptr};
+  bool bulk_tried[16] = {};
+  hipEvent_t ev[16][6] = {};
 };
 thread_local ThreadState g_ts;
 
@@ -92,7 +96,7 @@ void thread_release() {
   int cur = 0;
   (void)hipGetDevice(&cur);
   for (int d = 0; d < 16; ++d) {
-    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d]) continue;
+    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d]) continue;
     if (hipSetDevice(d) != hipSuccess) continue;
     if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
     if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
@@ -101,12 +105,18 @@ void thread_release() {
       (void)hipStreamSynchronize(g_ts.aux[d]);
       (void)hipStreamDestroy(g_ts.aux[d]);
     }
-    for (int e = 0; e < 4; ++e)
+    if (g_ts.bulk[d]) {
+      (void)hipStreamSynchronize(g_ts.bulk[d]);
+      (void)hipStreamDestroy(g_ts.bulk[d]);
+    }
+    for (int e = 0; e < 6; ++e)
       if (g_ts.ev[d][e]) (void)hipEventDestroy(g_ts.ev[d][e]);
     g_ts.ws[d] = Workspace();
     g_ts.stream[d] = nullptr;
     g_ts.aux[d] = nullptr;
-    for (int e = 0; e < 4; ++e) g_ts.ev[d][e] = nullptr;
+    g_ts.bulk[d] = nullptr;
+    g_ts.bulk_tried[d] = false;
+    for (int e = 0; e < 6; ++e) g_ts.ev[d][e] = nullptr;
   }
   (void)hipSetDevice(cur);
 }
@@ -193,9 +203,30 @@ hipStream_t aux_stream() {
   return g_ts.aux[d];
 }
 
+hipStream_t bulk_stream() {
+  const int d = current_device();
+  if (!g_ts.bulk_tried[d]) {
+    g_ts.bulk_tried[d] = true;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess)
+      return nullptr;
+    if (ncu <= 2 * BULK_RESERVED_CUS) return nullptr;
+    uint32_t mask[32];
+    const int words = (ncu + 31) / 32;
+    if (words > 32) return nullptr;
+    for (int i = 0; i < words; ++i) mask[i] = 0xffffffffu;
+    if (ncu % 32) mask[words - 1] = (1u << (ncu % 32)) - 1;
+    for (int c = 0; c < BULK_RESERVED_CUS; ++c) mask[c / 32] &= ~(1u << (c % 32));
+    hipStream_t st = nullptr;
+    if (hipExtStreamCreateWithCUMask(&st, (uint32_t)words, mask) != hipSuccess) return nullptr;
+    g_ts.bulk[d] = st;
+  }
+  return g_ts.bulk[d];
+}
+
 hipEvent_t sync_event(int i) {
   const int d = current_device();
-  if (i < 0 || i >= 4) return nullptr;
+  if (i < 0 || i >= 6) return nullptr;
   if (!g_ts.ev[d][i]) {
     hipEvent_t e = nullptr;
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;

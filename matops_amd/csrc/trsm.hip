@@ -136,14 +136,25 @@ __global__ __launch_bounds__(1024) void k_trsm_skinny(int N, int nrhs, int eff_l
     }
   };
   // X block c (rows of block c, this group's columns) into Xs[k][col]
+  // (all loads in flight before the first LDS write: one memory latency)
   auto load_x = [&](int c) {
     const int nk = min(SK_BS, N - c * SK_BS);
-    for (int idx = t; idx < SK_BS * SK_CG; idx += 1024) {
+    constexpr int XPER = SK_BS * SK_CG / 1024;
+    double v[XPER];
+#pragma unroll
+    for (int q = 0; q < XPER; ++q) {
+      const int idx = t + q * 1024;
       const int k = idx % SK_BS, col = idx / SK_BS;
-      Xs[k * SK_LD + col] = (k < nk && col < ncols)
-          ? __hip_atomic_load(&B[(long)c * SK_BS + k + (long)(c0 + col) * ldb], __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT)
-          : 0.0;
+      v[q] = (k < nk && col < ncols)
+                 ? __hip_atomic_load(&B[(long)c * SK_BS + k + (long)(c0 + col) * ldb],
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < XPER; ++q) {
+      const int idx = t + q * 1024;
+      const int k = idx % SK_BS, col = idx / SK_BS;
+      Xs[k * SK_LD + col] = v[q];
     }
   };
   // tile (ti, tj) of S (64 x 64, [row][k]) times Xs ([k][col])
