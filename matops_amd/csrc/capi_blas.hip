@@ -67,6 +67,7 @@ int mops_d_dgemm(int flags, int M, int N, int P, double alpha, const double* A, 
   if (lda < (arows > 1 ? arows : 1)) return -7;
   if (ldb < (brows > 1 ? brows : 1)) return -9;
   if (ldc < (M > 1 ? M : 1)) return -12;
+  MOPS_TRY(ws_reserve(gemm_split_ws_bytes(M, N, P, sizeof(double))));
   return gemm_d(as_stream(stream), ta, tb, M, N, P, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
@@ -90,7 +91,8 @@ int mops_dmult_mm_blocked3(mops_mdata_t* C, const mops_mdata_t* A, const mops_md
     return ST_HIP;
   }
   const size_t na = (size_t)rows * P, nbb = (size_t)P * cols, nc = (size_t)rows * cols;
-  MOPS_TRY(ws_reserve((na + nbb + nc) * sizeof(double) + 1024));
+  MOPS_TRY(ws_reserve((na + nbb + nc) * sizeof(double) + 1024 +
+                      gemm_split_ws_bytes(rows, cols, P, sizeof(double))));
   double* dA = (double*)ws_carve(na * sizeof(double) + 8);
   double* dB = (double*)ws_carve(nbb * sizeof(double) + 8);
   double* dC = (double*)ws_carve(nc * sizeof(double) + 8);
@@ -184,7 +186,8 @@ int mops_dmmat_rank_blk(mops_mdata_t* C, const mops_mdata_t* A, double alpha, do
     return ST_HIP;
   }
   const size_t na = (size_t)n * P, nc = (size_t)n * n;
-  MOPS_TRY(ws_reserve((na + nc) * sizeof(double) + 1024));
+  MOPS_TRY(ws_reserve((na + nc) * sizeof(double) + 1024 +
+                      gemm_split_ws_bytes(n, n, P, sizeof(double))));
   double* dA = (double*)ws_carve(na * sizeof(double));
   double* dC = (double*)ws_carve(nc * sizeof(double));
   // rows S:E of A (NOTRANS) or columns S:E (TRANSA), as dmmat_rank.c:83-124

@@ -127,7 +127,10 @@ __global__ __launch_bounds__(256) void k_adj_symmetrize(int n, double* A, long l
 size_t chol_adjoint_ws_bytes(int n, int nb) {
   if (n <= 0) return 0;
   if (nb <= 0 || nb > ADJ_NB) nb = ADJ_NB;
-  return (size_t)ADJ_NB * ADJ_NB * sizeof(double) + trsm_ws_bytes(F_RIGHT, nb, n) + 1024;
+  // the TN products of the sweep may split their long k (output 64 x n, k < n)
+  const size_t split = gemm_split_ws_bytes(ADJ_NB, n, n, sizeof(double)) +
+                       gemm_split_ws_bytes(ADJ_NB, ADJ_NB, n, sizeof(double));
+  return (size_t)ADJ_NB * ADJ_NB * sizeof(double) + trsm_ws_bytes(F_RIGHT, nb, n) + split + 1024;
 }
 
 int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, double* Ab,

@@ -63,6 +63,12 @@ struct GemmArgs {
   int tiles_m, tiles_n;
   long sA, sB, sC;     // element strides between batch entries (blockIdx.y)
   long sA2, sB2, sC2;  // second batch dimension (blockIdx.z)
+  // split of k (kchunk > 0, never with batches): blockIdx.y = chunk; the
+  // chunks before the last leave their partial tiles in part and raise
+  // kflags; the last chunk adds them in chunk order and writes C
+  int kchunk;
+  T* part;
+  int* kflags;
 };
 
 constexpr int GROUP_M = 8;
@@ -142,6 +148,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
     default: break;
   }
   kbeg = (kbeg / BK) * BK;
+  if (g.kchunk > 0) {
+    kbeg = blockIdx.y * g.kchunk;
+    kend = min(g.K, kbeg + g.kchunk);
+  }
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -262,6 +272,51 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
     }
   }
 
+  if (g.kchunk > 0) {
+    // partial tiles in the register layout (coalesced both ways); the
+    // reducing chunk is the last in dispatch order, so the chunks it waits
+    // for were all dispatched before it and never wait themselves
+    const int z = blockIdx.y, S = gridDim.y;
+    constexpr int TILE = BM * BN;
+    const long tid_tile = (long)tm * g.tiles_n + tn;  // slot of this output tile
+    const long ntile = (long)g.tiles_m * g.tiles_n;
+    if (z < S - 1) {
+      T* P = g.part + ((long)z * ntile + tid_tile) * TILE;
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) P[((i * FN + j) * 4 + r) * NT + tid] = acc[i][j][r];
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_store(&g.kflags[tid_tile * S + z], 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    if (tid == 0) {
+      for (int zz = 0; zz < S - 1; ++zz) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(&g.kflags[tid_tile * S + zz], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+               ++spins < (1u << 26))
+          __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    for (int zz = 0; zz < S - 1; ++zz) {
+      const T* P = g.part + ((long)zz * ntile + tid_tile) * TILE;
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += P[((i * FN + j) * 4 + r) * NT + tid];
+    }
+  }
+
   // ---- epilogue: lane holds C[m0+..+lr][n0+..+drow(lk, r)] ----
   // Processed EPI row blocks at a time: all loads of C for the chunk are
   // issued before its first store, so their latencies overlap (a load
@@ -301,6 +356,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
         }
     }
   }
+}
+
+__global__ void k_zero_flags_i32(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 template <typename T>
@@ -430,6 +489,32 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   g.sA2 = sA2;
   g.sB2 = sB2;
   g.sC2 = sC2;
+  g.kchunk = 0;
+  g.part = nullptr;
+  g.kflags = nullptr;
+  // Few output tiles and a long k: split k so that the grid still covers
+  // the GPU (the partial tiles are added in a fixed order, so the result
+  // does not depend on timing). Needs workspace; without it, no split.
+  int S = 0, kc = 0;
+  if ((mode == GM_FULL || mode == GM_LOWER || mode == GM_UPPER) && tri == TRI_NONE &&
+      batch == 1 && batch2 == 1 && gemm_split_k(M, N, K, S, kc)) {
+    const size_t mark = ws_mark();
+    const long tiles = (long)ceil_div(M, 64) * ceil_div(N, 64);
+    T* part = (T*)ws_carve((size_t)(S - 1) * tiles * 64 * 64 * sizeof(T));
+    int* kflags = (int*)ws_carve((size_t)tiles * S * sizeof(int));
+    if (part && kflags) {
+      hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(tiles * S)), dim3(256), 0, s, kflags,
+                         (int)(tiles * S));
+      MOPS_CHECK_HIP(hipGetLastError());
+      g.kchunk = kc;
+      g.part = part;
+      g.kflags = kflags;
+      const int st = launch_gemm<T, 64, 64, 16, 2, 2>(s, ta, tb, g, S, 1);
+      ws_release(mark);
+      return st;
+    }
+    ws_release(mark);
+  }
   // Large outputs: 128x128 tiles (8 MFMA blocks per wave, 8 waves). Smaller
   // ones, and the triangle-only updates (whose diagonal tiles do wasted
   // work and whose grids balance better in finer pieces), use 64x64 tiles.

@@ -94,7 +94,7 @@ size_t gesv_ir_ws_bytes(int n, int nrhs, int prec) {
   b += 2 * (size_t)n * nrhs * sizeof(double);         // R, D
   b += (size_t)n * sizeof(int) + 4 * nrhs * sizeof(double) + 4096;
   const size_t f = prec == 1 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
-  const size_t r = getrs_ws_bytes(n, nrhs);
+  const size_t r = getrs_ws_bytes(n, nrhs) + gemm_split_ws_bytes(n, nrhs, n, sizeof(double));
   return b + (f > r ? f : r);
 }
 

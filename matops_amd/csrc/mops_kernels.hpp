@@ -26,6 +26,34 @@ inline bool gemm_big_tiles(long M, long N, long batch = 1) {
   return ((M + 127) / 128) * ((N + 127) / 128) * batch >= 512;
 }
 
+// Split of k for outputs of few 64x64 tiles and a long k (fewer than 128
+// tiles and k >= 512, or fewer than 512 tiles and k >= 2048): S chunks of
+// kc (a multiple of 16, at least 128) so that S x tiles comes near 1024
+// workgroups. Returns false when no split applies.
+inline bool gemm_split_k(long M, long N, long K, int& S, int& kc) {
+  const long tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  if (tiles <= 0 || tiles >= 512 || K < 512) return false;
+  if (tiles >= 128 && K < 2048) return false;
+  long s = (1024 + tiles - 1) / tiles;
+  if (s > K / 128) s = K / 128;
+  if (s > 32) s = 32;
+  if (s < 2) return false;
+  long c = (K + s - 1) / s;
+  c = (c + 15) / 16 * 16;
+  S = (int)((K + c - 1) / c);
+  kc = (int)c;
+  return S >= 2;
+}
+
+// Workspace the split of gemm_d / gemm_s may carve (bytes, for elements of
+// elem_bytes); 0 when the shape is not split.
+inline size_t gemm_split_ws_bytes(long M, long N, long K, size_t elem_bytes) {
+  int S = 0, kc = 0;
+  if (!gemm_split_k(M, N, K, S, kc)) return 0;
+  const long tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  return (size_t)(S - 1) * tiles * 64 * 64 * elem_bytes + (size_t)tiles * S * sizeof(int) + 1024;
+}
+
 // C(MxN) = alpha*op(A)*op(B) + beta*C, column-major. beta == 0 never reads C
 // (NaN/Inf in C are cleared, as dscale_tile does at calgo/vec_scale.c:114).
 int gemm_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,

@@ -55,6 +55,45 @@ def test_mult_size_mismatch_and_empty():
     assert mo.Mult(np.zeros((0, 5), order="F"), E, fmat(3, 5, 4), 1.0, 0.0, 0) is None
 
 
+@pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 4000), (100, 300, 5000), (1, 1, 2048), (700, 65, 777),
+                                   (1000, 520, 1024)])
+def test_mult_long_k(flags, m, n, k):
+    """Few output tiles and a long inner dimension: k is split across
+    workgroups and the partial tiles are added in a fixed order (twice the
+    same call gives the same bits)."""
+    ta, tb = bool(flags & mo.TRANSA), bool(flags & mo.TRANSB)
+    A = fmat(*((k, m) if ta else (m, k)), seed=21)
+    B = fmat(*((n, k) if tb else (k, n)), seed=22)
+    C0 = fmat(m, n, seed=23)
+    outs = []
+    for _ in range(2):
+        C = C0.copy(order="F")
+        assert mo.Mult(C, A, B, -1.0, 0.5, flags) is None
+        outs.append(C)
+    ref = 0.5 * C0 - _op(A, ta) @ _op(B, tb)
+    bound = gemm_bound(np.abs(_op(A, ta)), np.abs(_op(B, tb)), k) + 2 * EPS * np.abs(0.5 * C0)
+    assert np.all(np.abs(outs[0] - ref) <= bound)
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("flags", [mo.LOWER, mo.UPPER, mo.LOWER | mo.TRANSA])
+def test_rankupdatesym_long_k(flags):
+    n, k = 64, 4000
+    A = fmat(*((k, n) if flags & mo.TRANSA else (n, k)), seed=31)
+    C0 = fmat(n, n, seed=32)
+    C = C0.copy(order="F")
+    mo.RankUpdateSym(C, A, -1.0, 1.0, flags)
+    AA = A.T @ A if flags & mo.TRANSA else A @ A.T
+    ref = C0 - AA
+    idx = np.tril_indices(n) if flags & mo.LOWER else np.triu_indices(n)
+    other = np.triu_indices(n, 1) if flags & mo.LOWER else np.tril_indices(n, -1)
+    bound = k * EPS * (np.abs(A.T) @ np.abs(A) if flags & mo.TRANSA else np.abs(A) @ np.abs(A.T)) \
+        + 2 * EPS * np.abs(C0)
+    assert np.all((np.abs(C - ref) <= bound)[idx])
+    np.testing.assert_array_equal(C[other], C0[other])
+
+
 def test_mult_dgemm_1024():
     n = 1024
     A, B = fmat(n, n, 1), fmat(n, n, 2)
