@@ -72,6 +72,7 @@ struct GemmArgs {
 };
 
 constexpr int GROUP_M = 8;
+constexpr int SMALL_KS = 2;  // k shares of the 64x64-tile kernel for full outputs (8 waves)
 constexpr int SG = 8;
 constexpr int NXCD = 8;
 
@@ -85,9 +86,13 @@ __device__ __forceinline__ int xcd_linear(int bid, int nblk) {
 
 // TRI: instance for the triangle-only modes (GM_LOWER / GM_UPPER), which
 // keeps the symmetric rank-k updates apart from the other GEMMs in profiles.
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool TA, bool TB, bool TRI>
-__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
-  constexpr int NT = 64 * WGM * WGN;
+// KS: waves per output sub-tile, splitting each slab's k steps between them
+// (their partial tiles are added through LDS at the end): more waves per
+// SIMD for small tiles without more workgroups.
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, int KS, bool TA, bool TB, bool TRI>
+__global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
+  constexpr int NTS = 64 * WGM * WGN;  // threads of one k share
+  constexpr int NT = NTS * KS;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
   // Row pitch = 128 bytes modulo 256 (doubles), so the two k-rows read by
@@ -100,8 +105,12 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
   static_assert(A_PP * NT * 2 == BM * BK && B_PP * NT * 2 == BN * BK, "tile");
   typedef typename Mfma<T>::acc_t acc_t;
 
-  __shared__ T As[2][BK * LDS_A];
-  __shared__ T Bs[2][BK * LDS_B];
+  static_assert(BK % (4 * KS) == 0, "k split");
+  constexpr int SMEM_AB = 2 * BK * (LDS_A + LDS_B);
+  constexpr int SMEM_RED = (KS - 1) * BM * BN;
+  __shared__ T smem[SMEM_AB > SMEM_RED ? SMEM_AB : SMEM_RED];
+  T(*As)[BK * LDS_A] = reinterpret_cast<T(*)[BK * LDS_A]>(smem);
+  T(*Bs)[BK * LDS_B] = reinterpret_cast<T(*)[BK * LDS_B]>(smem + 2 * BK * LDS_A);
 
   // ---- tile selection ----
   const int lin = xcd_linear(blockIdx.x, gridDim.x);
@@ -155,7 +164,10 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WGM, wn = wave / WGM;
+  const int kh = wave / (WGM * WGN), wsp = wave % (WGM * WGN);  // k share, sub-tile
+  const int wm = wsp % WGM, wn = wsp / WGM;
+  const int stid = wsp * 64 + lane;  // thread index within the k share
+  const bool lead = kh == 0;         // the k share that finishes the tile
   const int lr = lane & 15, lk = lane >> 4;
 
   acc_t acc[FM][FN];
@@ -255,7 +267,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
       const T* as = &As[buf][0];
       const T* bs = &Bs[buf][0];
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
+      for (int kk = kh * (BK / KS); kk < (kh + 1) * (BK / KS); kk += 4) {
         T af[FM], bf[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) af[i] = as[(kk + lk) * LDS_A + wm * WM + i * 16 + lr];
@@ -272,6 +284,33 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
     }
   }
 
+  if constexpr (KS > 1) {
+    // the k shares' partial tiles through LDS (free after the loop's last
+    // barrier), added by the lead share
+    __syncthreads();
+    if (!lead) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            smem[((((kh - 1) * FM + i) * FN + j) * 4 + r) * NTS + stid] = acc[i][j][r];
+    }
+    __syncthreads();
+    if (lead) {
+#pragma unroll
+      for (int h = 1; h < KS; ++h)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              acc[i][j][r] += smem[((((h - 1) * FM + i) * FN + j) * 4 + r) * NTS + stid];
+    }
+  }
+
   if (g.kchunk > 0) {
     // partial tiles in the register layout (coalesced both ways); the
     // reducing chunk is the last in dispatch order, so the chunks it waits
@@ -282,12 +321,14 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
     const long ntile = (long)g.tiles_m * g.tiles_n;
     if (z < S - 1) {
       T* P = g.part + ((long)z * ntile + tid_tile) * TILE;
+      if (lead) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
+          for (int j = 0; j < FN; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) P[((i * FN + j) * 4 + r) * NT + tid] = acc[i][j][r];
+            for (int r = 0; r < 4; ++r) P[((i * FN + j) * 4 + r) * NTS + stid] = acc[i][j][r];
+      }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0)
@@ -306,16 +347,20 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm(GemmArgs<T> g) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    for (int zz = 0; zz < S - 1; ++zz) {
-      const T* P = g.part + ((long)zz * ntile + tid_tile) * TILE;
+    if (lead) {
+      for (int zz = 0; zz < S - 1; ++zz) {
+        const T* P = g.part + ((long)zz * ntile + tid_tile) * TILE;
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
+          for (int j = 0; j < FN; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) acc[i][j][r] += P[((i * FN + j) * 4 + r) * NT + tid];
+            for (int r = 0; r < 4; ++r) acc[i][j][r] += P[((i * FN + j) * 4 + r) * NTS + stid];
+      }
     }
   }
+
+  if (!lead) return;  // no barrier follows
 
   // ---- epilogue: lane holds C[m0+..+lr][n0+..+drow(lk, r)] ----
   // Processed EPI row blocks at a time: all loads of C for the chunk are
@@ -390,7 +435,7 @@ inline int grid_for(long total) {
   return (int)b;
 }
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, int KS>
 int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int batch2) {
   g.tiles_m = ceil_div(g.M, BM);
   g.tiles_n = ceil_div(g.N, BN);
@@ -406,13 +451,13 @@ int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int b
     set_error("gemm: grid too large");
     return ST_BADARG;
   }
-  dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)batch2), block(64 * WGM * WGN);
+  dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)batch2), block(64 * WGM * WGN * KS);
   const bool tri = g.mode == GM_LOWER || g.mode == GM_UPPER;
 #define MOPS_GEMM_LAUNCH(TA_, TB_)                                                              \
   if (tri)                                                                                     \
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, TA_, TB_, true>), grid, block, 0, s, g); \
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, KS, TA_, TB_, true>), grid, block, 0, s, g); \
   else                                                                                         \
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, TA_, TB_, false>), grid, block, 0, s, g);
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, KS, TA_, TB_, false>), grid, block, 0, s, g);
   if (!ta && !tb) {
     MOPS_GEMM_LAUNCH(false, false)
   } else if (ta && !tb) {
@@ -509,7 +554,9 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
       g.kchunk = kc;
       g.part = part;
       g.kflags = kflags;
-      const int st = launch_gemm<T, 64, 64, 16, 2, 2>(s, ta, tb, g, S, 1);
+      const int st = (mode == GM_FULL)
+                         ? launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, S, 1)
+                         : launch_gemm<T, 64, 64, 16, 2, 2, 1>(s, ta, tb, g, S, 1);
       ws_release(mark);
       return st;
     }
@@ -517,10 +564,15 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   }
   // Large outputs: 128x128 tiles (8 MFMA blocks per wave, 8 waves). Smaller
   // ones, and the triangle-only updates (whose diagonal tiles do wasted
-  // work and whose grids balance better in finer pieces), use 64x64 tiles.
+  // work and whose grids balance better in finer pieces), use 64x64 tiles;
+  // full and trapezoid outputs with two k shares (8 waves per tile).
   if (mode != GM_LOWER && mode != GM_UPPER && gemm_big_tiles(M, N, (long)batch * batch2))
-    return launch_gemm<T, 128, 128, 16, 2, 4>(s, ta, tb, g, batch, batch2);
-  return launch_gemm<T, 64, 64, 16, 2, 2>(s, ta, tb, g, batch, batch2);
+    return launch_gemm<T, 128, 128, 16, 2, 4, 1>(s, ta, tb, g, batch, batch2);
+  // triangle-only updates keep one k share: with 4 workgroups of 4 waves per
+  // CU their large grids are faster than with 2 of 8 (measured at 4000-8000)
+  if (mode == GM_LOWER || mode == GM_UPPER)
+    return launch_gemm<T, 64, 64, 16, 2, 2, 1>(s, ta, tb, g, batch, batch2);
+  return launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, batch, batch2);
 }
 
 }  // namespace

@@ -126,7 +126,7 @@ def _tri(n, flags, seed):
 
 
 @pytest.mark.parametrize("flags", TRSM_FLAGS)
-@pytest.mark.parametrize("n,nrhs", [(7, 3), (96, 40), (700, 37)])
+@pytest.mark.parametrize("n,nrhs", [(7, 3), (96, 40), (700, 37), (700, 100), (1000, 130)])
 def test_solvetrm_all_variants(flags, n, nrhs):
     T, G = _tri(n, flags, 5)
     if flags & mo.UNIT:
