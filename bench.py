@@ -325,11 +325,29 @@ def extra_lu(L, ok, colmajor, torch, dev, sp, n=16384, nrhs=16, reps=3):
         raise RuntimeError("dgetrf: grid barrier timed out")
     eta = _eta(torch, A0, X, B0)
     flops = 2 * n ** 3 / 3 - n ** 2 / 2 + 5 * n / 6
-    return ({"n": n, "nrhs": nrhs, "factor_ms": round(best[0] * 1e3, 2),
-             "solve_ms": round(best[1] * 1e3, 2),
-             "total_ms": round((best[0] + best[1]) * 1e3, 2),
-             "factor_tflops": round(flops / best[0] / 1e12, 3),
-             "backward_error": eta}, X)
+    res = {"n": n, "nrhs": nrhs, "factor_ms": round(best[0] * 1e3, 2),
+           "solve_ms": round(best[1] * 1e3, 2),
+           "total_ms": round((best[0] + best[1]) * 1e3, 2),
+           "factor_tflops": round(flops / best[0] / 1e12, 3),
+           "backward_error": eta}
+    if reps > 0:
+        # one more factorization with HIP events around each panel
+        # factorization and each trailing GEMM (on the streams they run on)
+        A.copy_(A0)
+        L.mops_trace_enable(1)
+        torch.cuda.synchronize()
+        ok(L.mops_d_dgetrf(n, n, 256, A.data_ptr(), n, ipiv.data_ptr(), perm.data_ptr(),
+                           st.data_ptr(), sp), "dgetrf")
+        torch.cuda.synchronize()
+        for key, cls in (("trailing_gemm", 1), ("panel_factor", 2)):
+            cnt, ms, fl = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+            ok(L.mops_trace_collect(cls, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(fl)),
+               "trace")
+            res[key] = {"launches": cnt.value, "sum_ms": round(ms.value, 2)}
+            if fl.value > 0 and ms.value > 0:
+                res[key]["tflops"] = round(fl.value / ms.value / 1e9, 3)
+        L.mops_trace_enable(0)
+    return (res, X)
 
 
 def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, n=16384, nrhs=16, reps=2):

@@ -139,6 +139,7 @@ int mops_d_dsyrk(int flags, int N, int P, double alpha, const double* A, int lda
   if (lda < (arows > 1 ? arows : 1)) return -6;
   if (ldc < (N > 1 ? N : 1)) return -9;
   const int mode = (flags & F_LOWER) ? GM_LOWER : GM_UPPER;
+  MOPS_TRY(ws_reserve(gemm_split_ws_bytes(N, N, P, sizeof(double))));
   return gemm_d(as_stream(stream), ta, !ta, N, N, P, alpha, A, lda, A, lda, beta, C, ldc, mode);
 }
 
