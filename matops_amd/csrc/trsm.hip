@@ -72,29 +72,46 @@ __global__ __launch_bounds__(256) void k_trtri_leaf(int bb, int lower, int unit,
   }
 }
 
-// ---- LEFT solve for few right-hand sides: one kernel, blocks chained by flags
+// ---- LEFT solve for few right-hand sides: one kernel, blocks chained
 //
-// Workgroup (b, g), 1024 threads, solves block row b (64 rows) of
-// op(A) X = B for the column group g (64 columns); each of its 16 waves
-// owns one 16x16 tile of the block. With p the block solved just before b
-// (b-1 forward, b+1 backward) and D_b = op(inv(A_bb)):
-//   off the chain:  acc = B_b - sum_{c solved before p} op(A)_bc X_c
-//                   (each X_c as soon as its flag is up), y = D_b acc,
-//                   M_b = D_b op(A)_bp
-//   on the chain:   X_b = y - M_b X_p   (one 64x64x64 product)
-// X_b is written over B_b with agent-scope atomic stores, then flag (b, g)
-// goes up; readers load X with agent-scope atomic loads, so no cache
-// maintenance is needed. Workgroups are numbered in solve order, so every
-// block a workgroup waits for belongs to a workgroup dispatched before it.
-// The products run on the FP64 matrix cores (v_mfma_f64_16x16x4).
-constexpr int SK_BS = 64, SK_CG = 64;
-constexpr int SK_LD = SK_BS + 1;
-constexpr unsigned SK_SPIN_LIMIT = 1u << 24;
+// Workgroup (b, g), 256 threads, solves block row b (64 rows) of
+// op(A) X = B for the column group g (16 columns); wave w owns rows
+// 16w..16w+15 of the block. With p the block solved just before b (b-1
+// forward, b+1 backward), q the one before p, and D_b = inv(op(A)_bb):
+//   off the chain:  acc = B_b - sum_{c solved before p} op(A)_bc X_c,
+//                   y = D_b acc,   M_b = D_b op(A)_bp
+//   on the chain:   X_b = y - M_b X_p   (one 64x16x64 product)
+// X_b goes to B (the result) and to the workspace W, which was filled with
+// a signalling-NaN sentinel that no arithmetic result can equal. The
+// workgroups of the next two blocks (p = b and q = b) poll their X values
+// in W until no sentinel is left: the data is the signal, so a hand-off
+// costs one one-way store and one load round trip. Blocks solved earlier are
+// waited for on a flag per (block, group), raised once every W store of the
+// block has completed (one polled word instead of the whole block, for the
+// many workgroups that wait at the front of the chain). Every exchanged value
+// is an agent-scope atomic load or store, so no cache maintenance is needed.
+// Workgroups are numbered in solve order, so every block a workgroup waits
+// for belongs to a workgroup dispatched before it. The products run on the
+// FP64 matrix cores (v_mfma_f64_16x16x4): A operand lane l = (row l%16,
+// k l/16), B operand lane l = (k l/16, column l%16), result lane l, reg r =
+// (row l/16 + 4r, column l%16). D_b and M_b stay in registers in the A
+// operand layout (M_b computed as M_b^T = op(A)_bp^T D_b^T, whose result
+// layout is that A operand layout).
+constexpr int TC_BS = 64, TC_CG = 16, TC_T = 256, TC_KS = TC_BS / 4;
+constexpr int TC_LDA = TC_BS + 1;  // padded row of a 64x64 LDS block
+constexpr int TC_LDX = TC_CG + 1;  // padded row of a 64x16 LDS block
+constexpr unsigned long long TC_SENT = 0x7FF5A5A5A5A5A5A5ull;  // quiet bit clear
+constexpr unsigned long long TC_QNAN = 0x7FF8000000000000ull;
+constexpr unsigned TC_SPIN_LIMIT = 1u << 24;
 
-__device__ __forceinline__ void sk_wait(const int* flag, int* err) {
+__device__ __forceinline__ bool tc_is_sent(double v) {
+  return (unsigned long long)__double_as_longlong(v) == TC_SENT;
+}
+
+__device__ __forceinline__ void tc_wait_flag(const int* flag, int* err) {
   unsigned spins = 0;
   while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    if (++spins > SK_SPIN_LIMIT) {
+    if (++spins > TC_SPIN_LIMIT) {
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }
@@ -102,168 +119,228 @@ __device__ __forceinline__ void sk_wait(const int* flag, int* err) {
   }
 }
 
-__global__ __launch_bounds__(1024) void k_trsm_skinny(int N, int nrhs, int eff_lower, int trans,
-                                                      const double* __restrict__ A, long lda,
-                                                      const double* __restrict__ Dinv,
-                                                      double* B, long ldb, int* flags, int* err) {
-  __shared__ double As[SK_BS * SK_LD];  // op(A) block [row][k]
-  __shared__ double Xs[SK_BS * SK_LD];  // X block [k][col], or acc
-  __shared__ double Ds[SK_BS * SK_LD];  // D_b [row][k]
-  __shared__ double Ms[SK_BS * SK_LD];  // M_b [row][k]
-  const int nblk = (N + SK_BS - 1) / SK_BS;
-  const int ngrp = (nrhs + SK_CG - 1) / SK_CG;
+template <bool TRANS>
+__global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
+                                                     const double* __restrict__ A, long lda,
+                                                     const double* __restrict__ Dinv, double* B,
+                                                     long ldb, double* W, long ldw, int* flags,
+                                                     int* err) {
+  __shared__ double As[TC_BS * TC_LDA];  // op(A) block [row][k]
+  __shared__ double Xs[TC_BS * TC_LDX];  // X block [k][col]
+  __shared__ double Ys[TC_BS * TC_LDX];  // acc [k][col]
+  const int nblk = (N + TC_BS - 1) / TC_BS;
+  const int ngrp = (nrhs + TC_CG - 1) / TC_CG;
   const int step = blockIdx.x / ngrp, g = blockIdx.x % ngrp;
-  const int b = eff_lower ? step : nblk - 1 - step;
+  auto blk = [&](int it) { return fwd ? it : nblk - 1 - it; };  // block solved at step it
+  const int b = blk(step);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int li = lane & 15, lq = lane >> 4;
-  const int ti = w & 3, tj = w >> 2;  // this wave's tile: rows 16ti.., columns 16tj..
-  const int r0 = b * SK_BS, c0 = g * SK_CG;
-  const int nb_rows = min(SK_BS, N - r0);
-  const int ncols = min(SK_CG, nrhs - c0);
+  const int r0 = b * TC_BS, c0 = g * TC_CG;
+  const int nb_rows = min(TC_BS, N - r0);
+  const int ncols = min(TC_CG, nrhs - c0);
+  const int row_a = 16 * w + li;  // this lane's row in the A operand layout
 
-  // op(A) block (b, c) into S[i][k] (rows of block b, columns of block c)
-  auto load_opA = [&](double* S, int c) {
-    const int nk = min(SK_BS, N - c * SK_BS);
-    for (int idx = t; idx < SK_BS * SK_BS; idx += 1024) {
-      const int x = idx % SK_BS, y = idx / SK_BS;  // x fastest: contiguous in memory
-      if (!trans) {  // op(A)[i = x][k = y] = A[r0 + x][c*64 + y]
-        S[x * SK_LD + y] = (x < nb_rows && y < nk)
-                               ? A[(long)r0 + x + ((long)c * SK_BS + y) * lda] : 0.0;
-      } else {       // op(A)[i = y][k = x] = A[c*64 + x][r0 + y]
-        S[y * SK_LD + x] = (y < nb_rows && x < nk)
-                               ? A[(long)c * SK_BS + x + ((long)r0 + y) * lda] : 0.0;
+  // op(A) rows of block b, columns of block c, staged as As[i][k] (loads
+  // run along the contiguous dimension of A)
+  constexpr int SPER = TC_BS * TC_BS / TC_T;
+  auto stage_fetch = [&](int c, double* v) {
+    const int nk = min(TC_BS, N - c * TC_BS);
+#pragma unroll
+    for (int q = 0; q < SPER; ++q) {
+      const int idx = t + q * TC_T;
+      const int x = idx % TC_BS, y = idx / TC_BS;  // x runs along memory
+      if (!TRANS)  // op(A)[i = x][k = y] = A[r0 + x][c*64 + y]
+        v[q] = (x < nb_rows && y < nk) ? A[(long)r0 + x + ((long)c * TC_BS + y) * lda] : 0.0;
+      else         // op(A)[i = y][k = x] = A[c*64 + x][r0 + y]
+        v[q] = (y < nb_rows && x < nk) ? A[(long)c * TC_BS + x + ((long)r0 + y) * lda] : 0.0;
+    }
+  };
+  auto stage_store = [&](const double* v) {
+#pragma unroll
+    for (int q = 0; q < SPER; ++q) {
+      const int idx = t + q * TC_T;
+      const int x = idx % TC_BS, y = idx / TC_BS;
+      if (!TRANS)
+        As[x * TC_LDA + y] = v[q];
+      else
+        As[y * TC_LDA + x] = v[q];
+    }
+  };
+  // A operand of op(A)_bc for this wave straight from memory (no transpose
+  // needed: lanes run along the rows of A)
+  auto direct_fetch = [&](int c, double* a) {
+    const int nk = min(TC_BS, N - c * TC_BS);
+#pragma unroll
+    for (int s = 0; s < TC_KS; ++s) {
+      const int k = 4 * s + lq;
+      a[s] = (row_a < nb_rows && k < nk) ? A[(long)r0 + row_a + ((long)c * TC_BS + k) * lda] : 0.0;
+    }
+  };
+  // X block c (this group's columns) into Xs[k][col]; with poll, each value
+  // is loaded again until it is no longer the sentinel
+  auto load_x = [&](int c, bool poll) {
+    const int nk = min(TC_BS, N - c * TC_BS);
+    constexpr int XPER = TC_BS * TC_CG / TC_T;
+    double v[XPER];
+    bool need[XPER];
+#pragma unroll
+    for (int q = 0; q < XPER; ++q) {
+      const int idx = t + q * TC_T;
+      const int k = idx % TC_BS, col = idx / TC_BS;
+      need[q] = k < nk && col < ncols;
+      v[q] = need[q] ? __hip_atomic_load(&W[(long)c * TC_BS + k + (long)(c0 + col) * ldw],
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0.0;
+    }
+    if (poll) {
+      unsigned spins = 0;
+      for (;;) {
+        bool pending = false;
+#pragma unroll
+        for (int q = 0; q < XPER; ++q) pending |= need[q] && tc_is_sent(v[q]);
+        if (!pending) break;
+        if (++spins > TC_SPIN_LIMIT) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int q = 0; q < XPER; ++q) {
+          const int idx = t + q * TC_T;
+          const int k = idx % TC_BS, col = idx / TC_BS;
+          if (need[q] && tc_is_sent(v[q]))
+            v[q] = __hip_atomic_load(&W[(long)c * TC_BS + k + (long)(c0 + col) * ldw],
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
-  };
-  // X block c (rows of block c, this group's columns) into Xs[k][col]
-  // (all loads in flight before the first LDS write: one memory latency)
-  auto load_x = [&](int c) {
-    const int nk = min(SK_BS, N - c * SK_BS);
-    constexpr int XPER = SK_BS * SK_CG / 1024;
-    double v[XPER];
 #pragma unroll
     for (int q = 0; q < XPER; ++q) {
-      const int idx = t + q * 1024;
-      const int k = idx % SK_BS, col = idx / SK_BS;
-      v[q] = (k < nk && col < ncols)
-                 ? __hip_atomic_load(&B[(long)c * SK_BS + k + (long)(c0 + col) * ldb],
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                 : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < XPER; ++q) {
-      const int idx = t + q * 1024;
-      const int k = idx % SK_BS, col = idx / SK_BS;
-      Xs[k * SK_LD + col] = v[q];
+      const int idx = t + q * TC_T;
+      Xs[(idx % TC_BS) * TC_LDX + idx / TC_BS] = v[q];
     }
   };
-  // tile (ti, tj) of S (64 x 64, [row][k]) times Xs ([k][col])
-  auto mm = [&](const double* S) {
+  // this wave's 16x16 tile of (64 x 64 operand a, A layout) times S[k][col]
+  auto mm = [&](const double* a, const double* S) {
     v4d d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s4 = 0; s4 < SK_BS; s4 += 4)
-      d = __builtin_amdgcn_mfma_f64_16x16x4f64(S[(16 * ti + li) * SK_LD + s4 + lq],
-                                               Xs[(s4 + lq) * SK_LD + 16 * tj + li], d, 0, 0, 0);
+    for (int s = 0; s < TC_KS; ++s)
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], S[(4 * s + lq) * TC_LDX + li], d, 0, 0, 0);
     return d;
   };
 
-  // acc = B_b tile in the MFMA result layout: reg r <-> row 16ti + lq + 4r, col 16tj + li
+  // D_b in the A operand layout: dst[s] = op(D_b)[row_a][4s + lq]
+  double dst[TC_KS];
+  {
+    const double* D = Dinv + (long)b * TC_BS * TC_BS;
+#pragma unroll
+    for (int s = 0; s < TC_KS; ++s) {
+      const int k = 4 * s + lq;
+      dst[s] = (row_a < nb_rows && k < nb_rows)
+                   ? (TRANS ? D[k + (long)row_a * TC_BS] : D[row_a + (long)k * TC_BS]) : 0.0;
+    }
+  }
+  // acc = B_b tile in the result layout: reg r <-> row 16w + lq + 4r, column li
   v4d acc;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = 16 * ti + lq + 4 * r, col = 16 * tj + li;
-    acc[r] = (row < nb_rows && col < ncols) ? B[r0 + row + (long)(c0 + col) * ldb] : 0.0;
+    const int row = 16 * w + lq + 4 * r;
+    acc[r] = (row < nb_rows && li < ncols) ? B[r0 + row + (long)(c0 + li) * ldb] : 0.0;
   }
-  const int pred = eff_lower ? b - 1 : b + 1;  // the block solved just before b
-  const bool has_pred = step > 0;
-  // D_b (the inverse of A_bb, transposed for op = transpose) and, through
-  // Xs, M_b = D_b op(A)_{b,pred}: neither depends on any solution, so both
-  // are ready before the first wait
-  {
-    const double* D = Dinv + (long)b * SK_BS * SK_BS;
-    for (int idx = t; idx < SK_BS * SK_BS; idx += 1024) {
-      const int x = idx % SK_BS, y = idx / SK_BS;
-      const double v = (x < nb_rows && y < nb_rows) ? D[x + (long)y * SK_BS] : 0.0;
-      if (!trans)
-        Ds[x * SK_LD + y] = v;
-      else
-        Ds[y * SK_LD + x] = v;
-    }
-  }
-  if (has_pred) {
-    load_opA(Xs, pred);  // Xs[k][col] = op(A)_bp[k][col] (rows of b, columns of pred)
+  // M_b = D_b op(A)_bp, as M_b^T tiles t: result reg r of tile t is
+  // M_b[row_a][16t + lq + 4r] = the A operand value of k-step 4t + r
+  double mst[TC_KS];
+  if (step > 0) {
+    double v[SPER];
+    stage_fetch(blk(step - 1), v);
+    stage_store(v);
     __syncthreads();
-    const v4d m = mm(Ds);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) Ms[(16 * ti + lq + 4 * r) * SK_LD + 16 * tj + li] = m[r];
+    for (int tt = 0; tt < 4; ++tt) {
+      v4d d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < TC_KS; ++s)
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(4 * s + lq) * TC_LDA + 16 * tt + li], dst[s],
+                                                 d, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) mst[4 * tt + r] = d[r];
+    }
   }
-  // off the chain: every block solved before pred; the next op(A) block is
-  // fetched into registers while the current one is used
-  constexpr int PER = SK_BS * SK_BS / 1024;
-  double pre[PER];
-  auto fetch_opA = [&](int c) {
-    const int nk = min(SK_BS, N - c * SK_BS);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = t + q * 1024;
-      const int x = idx % SK_BS, y = idx / SK_BS;
-      pre[q] = !trans ? ((x < nb_rows && y < nk) ? A[(long)r0 + x + ((long)c * SK_BS + y) * lda] : 0.0)
-                      : ((y < nb_rows && x < nk) ? A[(long)c * SK_BS + x + ((long)r0 + y) * lda] : 0.0);
-    }
-  };
-  auto store_opA = [&]() {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int idx = t + q * 1024;
-      const int x = idx % SK_BS, y = idx / SK_BS;
-      if (!trans)
-        As[x * SK_LD + y] = pre[q];
-      else
-        As[y * SK_LD + x] = pre[q];
-    }
-  };
-  const int noff = step - 1;  // blocks solved before pred
-  if (noff > 0) fetch_opA(eff_lower ? 0 : nblk - 1);
+  // off the chain: every block solved before p, in solve order; the last of
+  // them (q) by polling its data, the others after their flag. The next
+  // block's op(A) values are fetched while the current one is used.
+  const int noff = step - 1;
+  double pre[TRANS ? SPER : TC_KS];
+  if (noff > 0) {
+    if (TRANS)
+      stage_fetch(blk(0), pre);
+    else
+      direct_fetch(blk(0), pre);
+  }
   for (int it = 0; it < noff; ++it) {
-    const int c = eff_lower ? it : nblk - 1 - it;
+    const int c = blk(it);
+    const bool last = it + 1 == noff;
+    double a[TC_KS];
     __syncthreads();  // As and Xs are free
-    store_opA();
-    if (it + 1 < noff) fetch_opA(eff_lower ? it + 1 : nblk - 2 - it);
-    if (t == 0) sk_wait(&flags[c * ngrp + g], err);
+    if (TRANS) {
+      stage_store(pre);
+    } else {
+#pragma unroll
+      for (int s = 0; s < TC_KS; ++s) a[s] = pre[s];
+    }
+    if (!last) {
+      if (TRANS)
+        stage_fetch(blk(it + 1), pre);
+      else
+        direct_fetch(blk(it + 1), pre);
+    }
+    if (!last) {
+      if (t == 0) tc_wait_flag(&flags[c * ngrp + g], err);
+      __syncthreads();
+    }
+    load_x(c, last);
     __syncthreads();
-    load_x(c);
-    __syncthreads();
-    const v4d d = mm(As);
+    if (TRANS) {
+#pragma unroll
+      for (int s = 0; s < TC_KS; ++s) a[s] = As[row_a * TC_LDA + 4 * s + lq];
+    }
+    const v4d d = mm(a, Xs);
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] -= d[r];
   }
   // y = D_b acc
-  __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; ++r) Xs[(16 * ti + lq + 4 * r) * SK_LD + 16 * tj + li] = acc[r];
-  __syncthreads();
-  v4d y = mm(Ds);
-  if (has_pred) {
-    // on the chain: X_b = y - M_b X_pred
+  for (int r = 0; r < 4; ++r) Ys[(16 * w + lq + 4 * r) * TC_LDX + li] = acc[r];
+  __syncthreads();  // also: every wave is done with Xs
+  v4d y = mm(dst, Ys);
+  if (step > 0) {
+    // on the chain: X_b = y - M_b X_p
+    load_x(blk(step - 1), true);
     __syncthreads();
-    if (t == 0) sk_wait(&flags[pred * ngrp + g], err);
-    __syncthreads();
-    load_x(pred);
-    __syncthreads();
-    const v4d z = mm(Ms);
+    const v4d z = mm(mst, Xs);
 #pragma unroll
     for (int r = 0; r < 4; ++r) y[r] -= z[r];
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = 16 * ti + lq + 4 * r, col = 16 * tj + li;
-    if (row < nb_rows && col < ncols)
-      __hip_atomic_store(&B[r0 + row + (long)(c0 + col) * ldb], y[r], __ATOMIC_RELAXED,
+    const int row = 16 * w + lq + 4 * r;
+    if (row < nb_rows && li < ncols) {
+      double v = y[r];
+      if (tc_is_sent(v)) v = __longlong_as_double((long long)TC_QNAN);
+      __hip_atomic_store(&W[r0 + row + (long)(c0 + li) * ldw], v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
+      B[r0 + row + (long)(c0 + li) * ldb] = v;
+    }
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (t == 0) __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0)
+    __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, long n, unsigned long long v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -388,8 +465,9 @@ size_t trsm_ws_bytes(int flags, int N, int nrhs) {
   size_t tri = trtri_core_ws(bs, nblk);
   size_t w = (size_t)N * nrhs * sizeof(double);
   const int nb64 = ceil_div(N, 64);
+  const int ngrp = ceil_div(nrhs, 16);
   size_t sk = (size_t)nb64 * 64 * 64 * sizeof(double) + (size_t)nb64 * 64 * sizeof(int) +
-              trtri_core_ws(64, nb64);
+              (size_t)nb64 * 64 * ngrp * 16 * sizeof(double) + trtri_core_ws(64, nb64);
   return (inv + tri + w > sk ? inv + tri + w : sk) + 6 * 256;
 }
 
@@ -418,20 +496,28 @@ int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const dou
   if (alpha != 1.0) MOPS_TRY(scale_tile_d(s, right ? nrhs : N, right ? N : nrhs, alpha, B, ldb));
   if (alpha == 0.0) return ST_OK;
 
-  if (!right && nrhs <= SKINNY_MAX_RHS && N > SK_BS) {
-    const int nb64 = ceil_div(N, SK_BS), ngrp = ceil_div(nrhs, SK_CG);
-    double* D = (double*)ws_carve((size_t)nb64 * SK_BS * SK_BS * sizeof(double));
+  if (!right && nrhs <= SKINNY_MAX_RHS && N > TC_BS) {
+    const int nb64 = ceil_div(N, TC_BS), ngrp = ceil_div(nrhs, TC_CG);
+    const long ldw = (long)nb64 * TC_BS;
+    double* D = (double*)ws_carve((size_t)nb64 * TC_BS * TC_BS * sizeof(double));
+    double* W = (double*)ws_carve((size_t)ldw * ngrp * TC_CG * sizeof(double));
     int* fl = (int*)ws_carve((size_t)(nb64 * ngrp + 1) * sizeof(int));
-    if (!D || !fl) {
+    if (!D || !W || !fl) {
       set_error("trsm: workspace too small");
       return ST_NOMEM;
     }
-    MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, SK_BS, A, lda, D));
+    MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, TC_BS, A, lda, D));
     hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nb64 * ngrp + 1, 256)), dim3(256), 0, s, fl,
                        nb64 * ngrp + 1);
-    hipLaunchKernelGGL(k_trsm_skinny, dim3(nb64 * ngrp), dim3(1024), 0, s, N, nrhs,
-                       eff_lower ? 1 : 0, trans ? 1 : 0, A, lda, D, B, ldb, fl,
-                       fl + nb64 * ngrp);
+    const long nw = ldw * ngrp * TC_CG;
+    hipLaunchKernelGGL(k_fill_u64, dim3((int)std::min<long>((nw + 255) / 256, 4096)), dim3(256), 0, s,
+                       (unsigned long long*)W, nw, TC_SENT);
+    if (trans)
+      hipLaunchKernelGGL(k_trsm_chain<true>, dim3(nb64 * ngrp), dim3(TC_T), 0, s, N, nrhs,
+                         eff_lower ? 1 : 0, A, lda, D, B, ldb, W, ldw, fl, fl + nb64 * ngrp);
+    else
+      hipLaunchKernelGGL(k_trsm_chain<false>, dim3(nb64 * ngrp), dim3(TC_T), 0, s, N, nrhs,
+                         eff_lower ? 1 : 0, A, lda, D, B, ldb, W, ldw, fl, fl + nb64 * ngrp);
     MOPS_CHECK_HIP(hipGetLastError());
     return ST_OK;
   }
