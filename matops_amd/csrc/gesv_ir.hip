@@ -22,13 +22,28 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
   atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-// out = max_i sum_j |A[i][j]| (the infinity norm); thread = row
-__global__ void k_inf_norm(int n, const double* __restrict__ A, long lda,
-                           unsigned long long* out) {
+// Infinity norm max_i sum_j |A[i][j]| in two passes: part[c][i] = sum of
+// |A[i][j]| over column chunk c (thread = row, workgroups over rows x
+// chunks), then the chunks of each row are added in chunk order (so the
+// value does not depend on timing) and the maximum is taken.
+constexpr int NORM_CHUNKS = 64;
+
+__global__ void k_row_abs_partial(int n, const double* __restrict__ A, long lda, int chunk,
+                                  double* part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = blockIdx.y * chunk, j1 = min(n, j0 + chunk);
+  double s = 0.0;
+  for (int j = j0; j < j1; ++j) s += fabs(A[i + (long)j * lda]);
+  part[(long)blockIdx.y * n + i] = s;
+}
+
+__global__ void k_inf_norm_finish(int n, int nch, const double* __restrict__ part,
+                                  unsigned long long* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += fabs(A[i + (long)j * lda]);
+  for (int c = 0; c < nch; ++c) s += part[(long)c * n + i];
   atomic_max_nonneg(out, s);
 }
 
@@ -93,6 +108,7 @@ size_t gesv_ir_ws_bytes(int n, int nrhs, int prec) {
   size_t b = (size_t)n * n * sizeof(double);          // LU
   b += 2 * (size_t)n * nrhs * sizeof(double);         // R, D
   b += (size_t)n * sizeof(int) + 4 * nrhs * sizeof(double) + 4096;
+  b += (size_t)NORM_CHUNKS * n * sizeof(double);     // row-sum chunks of the norm
   const size_t f = prec == 1 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
   const size_t r = getrs_ws_bytes(n, nrhs) + gemm_split_ws_bytes(n, nrhs, n, sizeof(double));
   return b + (f > r ? f : r);
@@ -118,7 +134,8 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
   double* cm = (double*)ws_carve(4 * (size_t)nrhs * sizeof(double) + 64);
   unsigned long long* anorm = (unsigned long long*)ws_carve(64);
   int* err = (int*)ws_carve(64);
-  if (!LU || !R || !D || !perm || !cm || !anorm || !err) {
+  double* part = (double*)ws_carve((size_t)NORM_CHUNKS * n * sizeof(double));
+  if (!LU || !R || !D || !perm || !cm || !anorm || !err || !part) {
     set_error("gesv_ir: workspace too small");
     return ST_NOMEM;
   }
@@ -146,7 +163,13 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
   }
   MOPS_CHECK_HIP(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, anorm);
-  hipLaunchKernelGGL(k_inf_norm, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, A, lda, anorm);
+  {
+    const int chunk = ceil_div(n, NORM_CHUNKS), nch = ceil_div(n, chunk);
+    hipLaunchKernelGGL(k_row_abs_partial, dim3(ceil_div(n, 256), nch), dim3(256), 0, s, n, A, lda,
+                       chunk, part);
+    hipLaunchKernelGGL(k_inf_norm_finish, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nch, part,
+                       anorm);
+  }
   hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, B, ldb, bmax);
   hipLaunchKernelGGL(k_zero_tile, dim3(grid_for((long)n * nrhs)), dim3(256), 0, s, n, nrhs, X, ldx);
   MOPS_CHECK_HIP(hipGetLastError());

@@ -176,3 +176,44 @@ def test_device_api_perm_and_pivots_agree():
     for k, q in enumerate(p):
         ident[[k, q]] = ident[[q, k]]
     assert np.array_equal(perm.cpu().numpy(), ident)
+
+
+def _bits(x):
+    return np.asarray(x, dtype=np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("case", ["nan_below", "nan_diag", "signalling_nan_rows"])
+def test_lu_nan_pivot_rules(case):
+    """pivotIndex (pivot.go:98-107): a NaN below the diagonal never wins, a
+    NaN on the diagonal is never replaced; pivots equal the oracle's. The
+    last case puts the signalling-NaN pattern 0x7FF5A5A5A5A5A5A5 into two
+    rows that are published as pivot candidates in the first column (by the
+    first and by the second workgroup of rows): their data must arrive
+    unchanged, and the untouched U row keeps the exact bits."""
+    n = 700
+    A0 = gi.mat(n, n, 4242, "normal")
+    sent = np.array([0x7FF5A5A5A5A5A5A5], dtype=np.uint64).view(np.float64)[0]
+    if case == "nan_below":
+        A0[50, 3] = np.nan
+        A0[600, 40] = np.nan
+    elif case == "nan_diag":
+        A0[7, 7] = np.nan
+    else:
+        A0[0, 0] = 50.0            # row 0 is the first pivot (and the old row k)
+        A0[600, 0] = 40.0          # the best candidate of the rows 512..1023
+        A0[0, 20] = sent
+        A0[600, 25] = sent
+    F = A0.copy(order="F")
+    piv = np.zeros(n, dtype=np.int64)
+    _, err = mo.DecomposeLU(F, piv, 64)
+    assert err is None
+    Fo = A0.copy(order="F")
+    pivo = np.zeros(n, dtype=np.int64)
+    O.use("port")
+    O.DecomposeLU(Fo, pivo, 64)
+    mism = np.nonzero(piv != pivo)[0]
+    assert mism.size == 0, f"first pivot mismatch at column {mism[:1]}"
+    if case == "signalling_nan_rows":
+        assert piv[0] == 0
+        assert _bits(F[0, 20]) == _bits(sent)
+        np.testing.assert_array_equal(_bits(np.delete(F[0], 20)), _bits(np.delete(A0[0], 20)))
