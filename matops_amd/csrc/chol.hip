@@ -647,13 +647,11 @@ int potrs_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda
             long ldb) {
   // SolveCHOL (chol.go:147-158): UPPER solves U^T then U; LOWER solves L then
   // L^T; neither flag leaves B unchanged.
-  if (flags & F_UPPER) {
-    MOPS_TRY(trsm_d(s, F_UPPER | F_TRANSA, N, nrhs, 1.0, A, lda, B, ldb));
-    MOPS_TRY(trsm_d(s, F_UPPER, N, nrhs, 1.0, A, lda, B, ldb));
-  } else if (flags & F_LOWER) {
-    MOPS_TRY(trsm_d(s, F_LOWER, N, nrhs, 1.0, A, lda, B, ldb));
-    MOPS_TRY(trsm_d(s, F_LOWER | F_TRANSA, N, nrhs, 1.0, A, lda, B, ldb));
-  }
+  // (both solves share the inverses of the diagonal blocks)
+  if (flags & F_UPPER)
+    return trsm_pair_d(s, F_UPPER | F_TRANSA, F_UPPER, N, nrhs, A, lda, B, ldb);
+  if (flags & F_LOWER)
+    return trsm_pair_d(s, F_LOWER, F_LOWER | F_TRANSA, N, nrhs, A, lda, B, ldb);
   return ST_OK;
 }
 

@@ -31,6 +31,12 @@ size_t trsm_ws_bytes(int flags, int N, int nrhs);
 int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
            double* B, long ldb);
 
+// Two LEFT solves with the same triangle of A (flags differing only in
+// TRANSA), alpha = 1, one after the other on B; with few right-hand sides
+// the inverses of the diagonal blocks are computed once for both.
+int trsm_pair_d(hipStream_t s, int flags1, int flags2, int N, int nrhs, const double* A,
+                long lda, double* B, long ldb);
+
 // Cholesky of the N x N matrix A in place (LOWER or UPPER flag), outer block
 // nb. `info` (device int, may be null) receives the 0-based global index of
 // the first NaN square root, or stays -1. `badval` (device double, may be

@@ -472,8 +472,73 @@ size_t trsm_ws_bytes(int flags, int N, int nrhs) {
 }
 
 namespace {
+// Workspace of the chained LEFT solve: the inverses of the 64x64 diagonal
+// blocks, the sentinel-filled copy of the solution and the flags.
+struct ChainWs {
+  double* D;
+  double* W;
+  int* fl;
+  long ldw;
+  int nb64, ngrp;
+};
+
+int chain_ws(int N, int nrhs, ChainWs& w) {
+  w.nb64 = ceil_div(N, TC_BS);
+  w.ngrp = ceil_div(nrhs, TC_CG);
+  w.ldw = (long)w.nb64 * TC_BS;
+  w.D = (double*)ws_carve((size_t)w.nb64 * TC_BS * TC_BS * sizeof(double));
+  w.W = (double*)ws_carve((size_t)w.ldw * w.ngrp * TC_CG * sizeof(double));
+  w.fl = (int*)ws_carve((size_t)(w.nb64 * w.ngrp + 1) * sizeof(int));
+  if (!w.D || !w.W || !w.fl) {
+    set_error("trsm: workspace too small");
+    return ST_NOMEM;
+  }
+  return ST_OK;
+}
+
+// op(A) X = B for a LEFT solve, with w.D holding the inverses of A's
+// diagonal blocks (of the triangle the flags select).
+int chain_solve(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda, double* B,
+                long ldb, const ChainWs& w) {
+  const bool trans = flags & F_TRANSA;
+  const bool eff_lower = ((flags & F_UPPER) != 0) == trans;  // op(A) is lower triangular
+  const int nflag = w.nb64 * w.ngrp + 1;
+  hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nflag, 256)), dim3(256), 0, s, w.fl, nflag);
+  const long nw = w.ldw * w.ngrp * TC_CG;
+  hipLaunchKernelGGL(k_fill_u64, dim3((int)std::min<long>((nw + 255) / 256, 4096)), dim3(256), 0,
+                     s, (unsigned long long*)w.W, nw, TC_SENT);
+  const dim3 grid(w.nb64 * w.ngrp), block(TC_T);
+  int* err = w.fl + w.nb64 * w.ngrp;
+  if (trans)
+    hipLaunchKernelGGL(k_trsm_chain<true>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
+                       w.D, B, ldb, w.W, w.ldw, w.fl, err);
+  else
+    hipLaunchKernelGGL(k_trsm_chain<false>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
+                       w.D, B, ldb, w.W, w.ldw, w.fl, err);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
 int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
               double* B, long ldb);
+}  // namespace
+
+int trsm_pair_d(hipStream_t s, int flags1, int flags2, int N, int nrhs, const double* A,
+                long lda, double* B, long ldb) {
+  const bool same_triangle = ((flags1 ^ flags2) & ~F_TRANSA) == 0;
+  if (!same_triangle || (flags1 & F_RIGHT) || nrhs > SKINNY_MAX_RHS || N <= TC_BS || nrhs <= 0) {
+    MOPS_TRY(trsm_d(s, flags1, N, nrhs, 1.0, A, lda, B, ldb));
+    return trsm_d(s, flags2, N, nrhs, 1.0, A, lda, B, ldb);
+  }
+  const size_t mark = ws_mark();
+  ChainWs w;
+  int st = chain_ws(N, nrhs, w);
+  if (st == ST_OK)
+    st = trtri_blocks_d(s, !(flags1 & F_UPPER), (flags1 & F_UNIT) != 0, N, TC_BS, A, lda, w.D);
+  if (st == ST_OK) st = chain_solve(s, flags1, N, nrhs, A, lda, B, ldb, w);
+  if (st == ST_OK) st = chain_solve(s, flags2, N, nrhs, A, lda, B, ldb, w);
+  ws_release(mark);
+  return st;
 }
 
 int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
@@ -497,29 +562,10 @@ int trsm_impl(hipStream_t s, int flags, int N, int nrhs, double alpha, const dou
   if (alpha == 0.0) return ST_OK;
 
   if (!right && nrhs <= SKINNY_MAX_RHS && N > TC_BS) {
-    const int nb64 = ceil_div(N, TC_BS), ngrp = ceil_div(nrhs, TC_CG);
-    const long ldw = (long)nb64 * TC_BS;
-    double* D = (double*)ws_carve((size_t)nb64 * TC_BS * TC_BS * sizeof(double));
-    double* W = (double*)ws_carve((size_t)ldw * ngrp * TC_CG * sizeof(double));
-    int* fl = (int*)ws_carve((size_t)(nb64 * ngrp + 1) * sizeof(int));
-    if (!D || !W || !fl) {
-      set_error("trsm: workspace too small");
-      return ST_NOMEM;
-    }
-    MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, TC_BS, A, lda, D));
-    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nb64 * ngrp + 1, 256)), dim3(256), 0, s, fl,
-                       nb64 * ngrp + 1);
-    const long nw = ldw * ngrp * TC_CG;
-    hipLaunchKernelGGL(k_fill_u64, dim3((int)std::min<long>((nw + 255) / 256, 4096)), dim3(256), 0, s,
-                       (unsigned long long*)W, nw, TC_SENT);
-    if (trans)
-      hipLaunchKernelGGL(k_trsm_chain<true>, dim3(nb64 * ngrp), dim3(TC_T), 0, s, N, nrhs,
-                         eff_lower ? 1 : 0, A, lda, D, B, ldb, W, ldw, fl, fl + nb64 * ngrp);
-    else
-      hipLaunchKernelGGL(k_trsm_chain<false>, dim3(nb64 * ngrp), dim3(TC_T), 0, s, N, nrhs,
-                         eff_lower ? 1 : 0, A, lda, D, B, ldb, W, ldw, fl, fl + nb64 * ngrp);
-    MOPS_CHECK_HIP(hipGetLastError());
-    return ST_OK;
+    ChainWs w;
+    MOPS_TRY(chain_ws(N, nrhs, w));
+    MOPS_TRY(trtri_blocks_d(s, !upper, unit, N, TC_BS, A, lda, w.D));
+    return chain_solve(s, flags, N, nrhs, A, lda, B, ldb, w);
   }
   const int bs = trsm_block(N);
   const int nblk = ceil_div(N, bs);
