@@ -130,7 +130,12 @@ size_t chol_adjoint_ws_bytes(int n, int nb) {
   // the TN products of the sweep may split their long k (output 64 x n, k < n)
   const size_t split = gemm_split_ws_bytes(ADJ_NB, n, n, sizeof(double)) +
                        gemm_split_ws_bytes(ADJ_NB, ADJ_NB, n, sizeof(double));
-  return (size_t)ADJ_NB * ADJ_NB * sizeof(double) + trsm_ws_bytes(F_RIGHT, nb, n) + split + 1024;
+  // the inverses of all diagonal blocks (and their construction), the
+  // solution of one block column
+  const size_t inv = (size_t)ceil_div(n, nb) * nb * nb * sizeof(double) +
+                     trtri_blocks_ws_bytes(n, nb);
+  const size_t w = (size_t)n * nb * sizeof(double);
+  return (size_t)ADJ_NB * ADJ_NB * sizeof(double) + inv + w + split + 4096;
 }
 
 int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, double* Ab,
@@ -138,9 +143,20 @@ int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, doub
   if (n <= 0) return ST_OK;
   if (nb <= 0 || nb > ADJ_NB) nb = ADJ_NB;
   double* S = (double*)ws_carve((size_t)ADJ_NB * ADJ_NB * sizeof(double));
-  if (!S) {
+  // inverses of every diagonal block of L, all in one launch up front (they
+  // depend on L only): each block step's triangular solve is then one GEMM
+  const int nblk = ceil_div(n, nb);
+  double* Dinv = (double*)ws_carve((size_t)nblk * nb * nb * sizeof(double));
+  double* W = (double*)ws_carve((size_t)n * nb * sizeof(double));
+  if (!S || !Dinv || !W) {
     set_error("chol_adjoint: workspace too small");
     return ST_NOMEM;
+  }
+  {
+    const size_t mark = ws_mark();
+    const int st = trtri_blocks_d(s, true, false, n, nb, L, ldl, Dinv);
+    ws_release(mark);
+    MOPS_TRY(st);
   }
   for (int J0 = ((n - 1) / nb) * nb; J0 >= 0; J0 -= nb) {
     const int J1 = min(J0 + nb, n), jb = J1 - J0, m = n - J1;
@@ -153,7 +169,11 @@ int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, doub
     double* Bb = Ab + J1;
     double* Cb = Ab + J1 + (long)J0 * lda;
     if (m > 0) {
-      MOPS_TRY(trsm_d(s, F_RIGHT | F_LOWER, jb, m, 1.0, D, ldl, Cb, lda));
+      // Cb = Cb D^-1
+      const double* Di = Dinv + (long)(J0 / nb) * nb * nb;
+      MOPS_TRY(gemm_d(s, false, false, m, jb, jb, 1.0, Cb, lda, Di, nb, 0.0, W, m, GM_FULL,
+                      TRI_B_LOWER));
+      MOPS_TRY(copy_tile_d(s, m, jb, W, m, Cb, lda));
       if (J0 > 0) MOPS_TRY(gemm_d(s, false, false, m, J0, jb, -1.0, Cb, lda, R, ldl, 1.0, Bb, lda));
       MOPS_TRY(gemm_d(s, true, false, jb, jb, m, -1.0, Cb, lda, C, ldl, 1.0, Db, lda, GM_LOWER));
     }

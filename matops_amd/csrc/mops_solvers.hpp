@@ -8,6 +8,9 @@ namespace mops {
 
 // Bytes of workspace trtri_d needs beyond X for an n x n triangle.
 size_t trtri_ws_bytes(int n);
+// Bytes of workspace trtri_blocks_d needs beyond X (the ceil(n/bs) inverses
+// of bs x bs blocks).
+size_t trtri_blocks_ws_bytes(int n, int bs);
 // X (n x n, dense, zeros in the other triangle) = inverse of the lower
 // (lower=true) or upper triangle of A; unit: the diagonal is taken as 1 and
 // never read. Inverts all 64x64 diagonal blocks in one launch, then combines

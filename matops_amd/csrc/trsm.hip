@@ -427,6 +427,12 @@ inline int trsm_block(int N) {
 
 size_t trtri_ws_bytes(int n) { return trtri_core_ws(n, 1) + 256; }
 
+size_t trtri_blocks_ws_bytes(int n, int bs) {
+  if (n <= 0 || bs <= 0) return 0;
+  const int nfull = n / bs;
+  return trtri_core_ws(bs, nfull > 0 ? nfull : 1) + 256;
+}
+
 int trtri_d(hipStream_t s, bool lower, bool unit, int n, const double* A, long lda, double* X,
             long ldx) {
   if (n <= 0) return ST_OK;
