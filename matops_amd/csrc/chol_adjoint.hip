@@ -158,6 +158,12 @@ int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, doub
     ws_release(mark);
     MOPS_TRY(st);
   }
+  hipStream_t s2 = aux_stream();
+  hipEvent_t e_db = sync_event(0), e_sweep = sync_event(1);
+  if (!s2 || !e_db || !e_sweep) {
+    set_error("chol_adjoint: could not create the second stream");
+    return ST_HIP;
+  }
   for (int J0 = ((n - 1) / nb) * nb; J0 >= 0; J0 -= nb) {
     const int J1 = min(J0 + nb, n), jb = J1 - J0, m = n - J1;
     const double* D = L + J0 + (long)J0 * ldl;
@@ -174,16 +180,22 @@ int chol_adjoint_d(hipStream_t s, int n, int nb, const double* L, long ldl, doub
       MOPS_TRY(gemm_d(s, false, false, m, jb, jb, 1.0, Cb, lda, Di, nb, 0.0, W, m, GM_FULL,
                       TRI_B_LOWER));
       MOPS_TRY(copy_tile_d(s, m, jb, W, m, Cb, lda));
-      if (J0 > 0) MOPS_TRY(gemm_d(s, false, false, m, J0, jb, -1.0, Cb, lda, R, ldl, 1.0, Bb, lda));
       MOPS_TRY(gemm_d(s, true, false, jb, jb, m, -1.0, Cb, lda, C, ldl, 1.0, Db, lda, GM_LOWER));
     }
-    hipLaunchKernelGGL(k_chol_adj_diag, dim3(1), dim3(256), 0, s, jb, D, ldl, Db, lda, S);
+    // The diagonal sweep (one workgroup, latency-bound) runs on the second
+    // stream beside the products that do not need its result; both become
+    // ready at the same moment, so the sweep gets its CU first.
+    MOPS_CHECK_HIP(hipEventRecord(e_db, s));
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_db, 0));
+    hipLaunchKernelGGL(k_chol_adj_diag, dim3(1), dim3(256), 0, s2, jb, D, ldl, Db, lda, S);
     MOPS_CHECK_HIP(hipGetLastError());
-    if (J0 > 0) {
-      if (m > 0)
-        MOPS_TRY(gemm_d(s, true, false, jb, J0, m, -1.0, Cb, lda, Bm, ldl, 1.0, Rb, lda));
-      MOPS_TRY(gemm_d(s, false, false, jb, J0, jb, -1.0, S, jb, R, ldl, 1.0, Rb, lda));
+    MOPS_CHECK_HIP(hipEventRecord(e_sweep, s2));
+    if (J0 > 0 && m > 0) {
+      MOPS_TRY(gemm_d(s, false, false, m, J0, jb, -1.0, Cb, lda, R, ldl, 1.0, Bb, lda));
+      MOPS_TRY(gemm_d(s, true, false, jb, J0, m, -1.0, Cb, lda, Bm, ldl, 1.0, Rb, lda));
     }
+    MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_sweep, 0));
+    if (J0 > 0) MOPS_TRY(gemm_d(s, false, false, jb, J0, jb, -1.0, S, jb, R, ldl, 1.0, Rb, lda));
   }
   const int nt = ceil_div(n, 32);
   hipLaunchKernelGGL(k_adj_symmetrize, dim3(nt, nt), dim3(256), 0, s, n, Ab, lda);
