@@ -598,36 +598,45 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
     // when the block column update is done, at the moment the next panel
     // becomes ready too: both kernels then enter the dispatcher together,
     // and the panel's workgroups are not left waiting for CU slots behind
-    // thousands of queued tiles.
+    // thousands of queued tiles. The rest update does block column k+2
+    // first (a trapezoid) and then the triangle beyond it: the next block
+    // column update needs only the first part, so the panel chain does not
+    // wait for the whole triangle.
     hipStream_t s2 = aux_stream();
-    hipEvent_t e_fork = sync_event(0), e_col = sync_event(1), e_rest = sync_event(2),
+    hipEvent_t e_fork = sync_event(0), e_col = sync_event(1), e_next = sync_event(2),
                e_join = sync_event(3);
-    if (!s2 || !e_fork || !e_col || !e_rest || !e_join) {
+    if (!s2 || !e_fork || !e_col || !e_next || !e_join) {
       set_error("potrf: could not create the look-ahead stream");
       return ST_HIP;
     }
     MOPS_CHECK_HIP(hipEventRecord(e_fork, s));
     MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_fork, 0));
     MOPS_TRY(chol_panel(s2, upper, N, 0, min(nb, N), A, lda, fl, info, badval));
-    bool have_rest = false;
+    bool have_next = false;
     for (int k = 0; k < N; k += nb) {
       const int kb = min(nb, N - k);
       const int m = N - k - kb;
       if (m <= 0) break;
       const int kb1 = min(nb, m);  // width of the next block column
-      // s2 may touch block column k+1 only after the previous rest update
-      if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
+      // s2 may touch block column k+1 only after step k-1 has updated it
+      if (have_next) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_next, 0));
       MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
       MOPS_CHECK_HIP(hipEventRecord(e_col, s2));  // after panel k and this update
+      have_next = false;
       if (m > kb1) {
         const int mr = m - kb1;
+        const int kb2 = min(nb, mr);  // block column k+2
         MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
-        // timed when tracing: every launch of the triangle-only GEMM
-        trace_mark_begin(s, TR_SYRK_UPDATE);
-        MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, mr, false, A, lda));
-        trace_mark_end(s, TR_SYRK_UPDATE, (double)mr * (mr + 1) * kb);
-        MOPS_CHECK_HIP(hipEventRecord(e_rest, s));
-        have_rest = true;
+        MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, kb2, true, A, lda));
+        MOPS_CHECK_HIP(hipEventRecord(e_next, s));
+        have_next = true;
+        if (mr > kb2) {
+          // timed when tracing: every launch of the triangle-only GEMM
+          const int mq = mr - kb2;
+          trace_mark_begin(s, TR_SYRK_UPDATE);
+          MOPS_TRY(chol_update(s, upper, N, k, kb, kb1 + kb2, mq, false, A, lda));
+          trace_mark_end(s, TR_SYRK_UPDATE, (double)mq * (mq + 1) * kb);
+        }
       }
       MOPS_TRY(chol_panel(s2, upper, N, k + kb, kb1, A, lda, fl, info, badval));
     }
