@@ -599,13 +599,13 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
     // becomes ready too: both kernels then enter the dispatcher together,
     // and the panel's workgroups are not left waiting for CU slots behind
     // thousands of queued tiles. The rest update does block column k+2
-    // first (a trapezoid) and then the triangle beyond it: the next block
-    // column update needs only the first part, so the panel chain does not
-    // wait for the whole triangle.
+    // first (a trapezoid, as soon as panel k is done) and then the triangle
+    // beyond it: the next block column update needs only the first part, so
+    // the panel chain does not wait for the whole triangle.
     hipStream_t s2 = aux_stream();
     hipEvent_t e_fork = sync_event(0), e_col = sync_event(1), e_next = sync_event(2),
-               e_join = sync_event(3);
-    if (!s2 || !e_fork || !e_col || !e_next || !e_join) {
+               e_join = sync_event(3), e_pan = sync_event(4);
+    if (!s2 || !e_fork || !e_col || !e_next || !e_join || !e_pan) {
       set_error("potrf: could not create the look-ahead stream");
       return ST_HIP;
     }
@@ -618,6 +618,7 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       const int m = N - k - kb;
       if (m <= 0) break;
       const int kb1 = min(nb, m);  // width of the next block column
+      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));  // panel k done
       // s2 may touch block column k+1 only after step k-1 has updated it
       if (have_next) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_next, 0));
       MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
@@ -626,10 +627,13 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       if (m > kb1) {
         const int mr = m - kb1;
         const int kb2 = min(nb, mr);  // block column k+2
-        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
+        // block column k+2 needs panel k only; the triangle waits for the
+        // block column update, to start together with the next panel
+        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
         MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, kb2, true, A, lda));
         MOPS_CHECK_HIP(hipEventRecord(e_next, s));
         have_next = true;
+        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
         if (mr > kb2) {
           // timed when tracing: every launch of the triangle-only GEMM
           const int mq = mr - kb2;
