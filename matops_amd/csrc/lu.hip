@@ -19,9 +19,12 @@
 // Panel (columns k..k+kb, rows k..m-1), right-looking in 32-column leaves.
 // A leaf is one cooperative kernel: G workgroups of 512 threads, one row
 // per thread held in registers. Per column every workgroup finds its best
-// candidate (wave shuffles), publishes it with the candidate row, and after
-// one grid barrier all of them agree on the pivot, exchange the two rows in
-// registers and apply the rank-1 update to their rows. The interchanges of
+// candidate (wave shuffles) and publishes it with the candidate row into
+// exchange records that start out as a sentinel; every workgroup polls the
+// candidates' keys until all are there, agrees on the pivot, loads the
+// pivot row and the old row k, exchanges the two rows in registers and
+// applies the rank-1 update to its rows (no grid barrier per column). The
+// interchanges of
 // the leaf's rows in the other panel columns are applied once, at the end
 // of the leaf, as a gather of the rows that moved. After each leaf the
 // U block right of it is solved with the leaf's unit triangle (one small
@@ -43,6 +46,11 @@ constexpr int LEAF_R = 512;   // rows per workgroup in a leaf (= threads)
 constexpr int LEAF_GMAX = 64;
 constexpr int LEAF_SW = LEAF_W + 3;  // exchange slot: value, row, row data, perm
 constexpr unsigned SPIN_LIMIT = 1u << 26;
+// Panel state (row movement, inverse of the unit triangle) rotates over
+// three slots: the rest update of step k may still read slot k mod 3 while
+// the panel chain factors panel k+2; panel k+3 reuses the slot only after
+// the bulk stream has finished step k.
+constexpr int LU_NPAR = 3;
 
 // ---- copy-transpose of a column-major tile (64x64 through LDS) ----
 template <typename S, typename D>
@@ -73,6 +81,18 @@ __device__ __forceinline__ void st_agent(double* p, double v) {
 __device__ __forceinline__ double ld_agent(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Exchange records start out as this signalling-NaN pattern (quiet bit
+// clear), which no arithmetic result can equal: a reader that finds any
+// other value in a word has the published one.
+constexpr unsigned long long XSENT = 0x7FF5A5A5A5A5A5A5ull;
+__device__ __forceinline__ bool is_xsent(double v) {
+  return (unsigned long long)__double_as_longlong(v) == XSENT;
+}
+// One column's exchange records: LEAF_GMAX candidate slots [key, row, row
+// data, perm], then the old row k [data, perm, key]. Columns use a ring of
+// LEAF_RING such areas.
+constexpr int LEAF_XAREA = LEAF_GMAX * LEAF_SW + LEAF_W + 2;
+constexpr int LEAF_RING = 3;
 
 struct LeafArgs {
   double* T;       // A^T, column-major: T[c + r*ldT] = A[r][c]
@@ -83,17 +103,18 @@ struct LeafArgs {
   int pc0, pc1;    // columns of the enclosing panel (rows swapped in them too)
   int* piv;        // global pivots, piv[c0 + j]
   int* perm;       // perm[i] = original row now at position i
-  double* xb;      // exchange slots: [2][G][LEAF_SW]
-  double* rowk;    // exchange: [2][LEAF_W + 1]
+  double* xa;      // exchange records: a ring of 3 areas of LEAF_XAREA doubles
+  int ring0;       // ring position of the leaf's first column
   double* stage;   // moved values of the other panel columns: [2 * LEAF_W * 256]
   unsigned* ctr;   // grid barrier counter (monotonic over the factorization)
   unsigned base;   // its value when this leaf starts
   int* err;        // set to 1 if a barrier wait timed out
 };
 
-// Grid barrier among the workgroups of a cooperative launch. Everything the
-// workgroups exchange inside the leaf is written and read with agent-scope
-// atomics (coherent without cache maintenance), so the barrier only has to
+// Grid barrier among the workgroups of a cooperative launch (the leaf's
+// final gather). Everything the workgroups exchange inside the leaf is
+// written and read with agent-scope atomics (coherent without cache
+// maintenance), so the barrier only has to
 // order those accesses: every thread's stores are complete (waitcnt) before
 // the workgroup barrier, the arrival is a relaxed atomic add, and the reads
 // that follow the spin depend on it. No L2 write-back or invalidate is
@@ -162,7 +183,8 @@ __device__ __forceinline__ int wave_first_max(double v, double& vmax) {
 // the panel move later, all at once, from the permutation).
 __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
   constexpr int W = LEAF_W, SW = LEAF_SW, NWV = LEAF_R / 64;
-  __shared__ double xs[LEAF_GMAX * SW + W + 1];  // slots, then the old row k
+  __shared__ double xs[2 * (W + 1)];  // the pivot row, then the old row k
+  __shared__ int pinfo;               // the pivot of the current column
   __shared__ double wv[NWV];
   __shared__ int wr[NWV];
   __shared__ double wrow[NWV][W + 1];  // each wave's best row (data, perm)
@@ -189,8 +211,12 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
 #pragma unroll 1
   for (int j = 0; j < w; ++j) {
     const int k = c0 + j;
-    double* xbj = a.xb + (long)(j & 1) * G * SW;
-    double* rkj = a.rowk + (j & 1) * (W + 1);
+    // This column's records are in ring area (ring0 + j) mod 3. An area is
+    // used again 3 columns later: each workgroup puts the sentinel back into
+    // its own slot (workgroup 0 also into the old row) two columns ahead,
+    // once every workgroup is done reading it.
+    double* xaj = a.xa + (long)((a.ring0 + j) % LEAF_RING) * LEAF_XAREA;
+    double* xareset = a.xa + (long)((a.ring0 + j + 2) % LEAF_RING) * LEAF_XAREA;
     // 1. candidate of each wave; the winning lane parks its row in LDS.
     // Keys: |a|; a NaN is -1 (never chosen) except on the diagonal, where
     // it is +inf (nothing replaces it); rows that are not candidates -2.
@@ -216,53 +242,112 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
       okl[W] = (double)pm;
     }
     __syncthreads();
-    // 2. publish this workgroup's candidate (and the old row k)
+    // 2. publish this workgroup's candidate (and the old row k). The key
+    // and the row index are never the sentinel; row data that happens to
+    // equal it is published "escaped": data first, then (after the stores
+    // completed) the row index stored as -row-1, so readers know the data
+    // was complete before the key. The wait at the start orders the
+    // sentinel this wave put back into a ring area before anything of this
+    // column (a reader of this column's records then finds either the
+    // sentinel or the new record there).
     if (wave == 0) {
+      __builtin_amdgcn_s_waitcnt(0);
       double bv;
       const int bq = wave_first_max(lane < NWV ? wv[lane] : -3.0, bv);
       const int br = wr[bq];
-      double* slot = xbj + (long)g * SW;
+      double* slot = xaj + (long)g * SW;
+      const double dv = lane <= W ? wrow[bq][lane] : 0.0;
+      const bool esc = __ballot(lane <= W && is_xsent(dv)) != 0;
+      if (lane <= W) st_agent(&slot[2 + lane], dv);
+      if (esc) __builtin_amdgcn_s_waitcnt(0);
       if (lane == 0) {
+        st_agent(&slot[1], esc ? -(double)br - 1.0 : (double)br);
         st_agent(&slot[0], bv);
-        st_agent(&slot[1], (double)br);
       }
-      if (br != INT_MAX && lane <= W) st_agent(&slot[2 + lane], wrow[bq][lane]);
     }
-    if (wave == 1 && k >= rg0 && k < rg0 + LEAF_R && lane <= W) st_agent(&rkj[lane], okl[lane]);
-    target += (unsigned)G;
-    grid_sync(a.ctr, target, a.err);
-    // 3. every candidate and the old row k, in one round trip (all loads
-    // issued before the first LDS write)
-    {
-      constexpr int XPT = (LEAF_GMAX * SW + LEAF_R - 1) / LEAF_R;
-      double xv[XPT];
-#pragma unroll
-      for (int q = 0; q < XPT; ++q) {
-        const int idx = t + q * LEAF_R;
-        xv[q] = idx < G * SW ? ld_agent(&xbj[idx]) : 0.0;
+    if (wave == 1 && k >= rg0 && k < rg0 + LEAF_R) {
+      double* rk = xaj + (long)LEAF_GMAX * SW;
+      const double dv = lane <= W ? okl[lane] : 0.0;
+      const bool esc = __ballot(lane <= W && is_xsent(dv)) != 0;
+      if (lane <= W) st_agent(&rk[lane], dv);
+      if (esc) __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0) st_agent(&rk[W + 1], esc ? 2.0 : 1.0);
+    }
+    // 3. wave 0 waits for every slot's key and row index and for the old
+    // row k's key (one slot per lane: a few words per workgroup are polled),
+    // chooses the pivot (the first workgroup holding the largest key;
+    // workgroups are in row order), then loads the pivot row and the old
+    // row k, again while they hold the sentinel unless escaped. One store
+    // and two load round trips per column, no grid barrier.
+    if (wave == 0) {
+      const bool has = lane < G;
+      double kv = -3.0, rv = 0.0, rkk = 1.0;
+      if (has) {
+        kv = ld_agent(&xaj[(long)lane * SW]);
+        rv = ld_agent(&xaj[(long)lane * SW + 1]);
       }
-      const bool rk_lane = t >= LEAF_R - (W + 1);
-      const double rk = rk_lane ? ld_agent(&rkj[t - (LEAF_R - (W + 1))]) : 0.0;
-#pragma unroll
-      for (int q = 0; q < XPT; ++q) {
-        const int idx = t + q * LEAF_R;
-        if (idx < G * SW) xs[idx] = xv[q];
+      if (lane == 0) rkk = ld_agent(&xaj[(long)LEAF_GMAX * SW + W + 1]);
+      unsigned spins = 0;
+      for (;;) {
+        const bool pend = (has && (is_xsent(kv) || is_xsent(rv))) || (lane == 0 && is_xsent(rkk));
+        if (__ballot(pend) == 0) break;
+        if (++spins > SPIN_LIMIT) {
+          if (lane == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (has && is_xsent(kv)) kv = ld_agent(&xaj[(long)lane * SW]);
+        if (has && is_xsent(rv)) rv = ld_agent(&xaj[(long)lane * SW + 1]);
+        if (lane == 0 && is_xsent(rkk)) rkk = ld_agent(&xaj[(long)LEAF_GMAX * SW + W + 1]);
       }
-      if (rk_lane) xs[G * SW + t - (LEAF_R - (W + 1))] = rk;
+      double bv;
+      const int bq = wave_first_max(kv, bv);
+      const double brw = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rv), bq),
+                                          __builtin_amdgcn_readlane(__double2loint(rv), bq));
+      const double rk_key = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rkk), 0),
+                                             __builtin_amdgcn_readlane(__double2loint(rkk), 0));
+      const bool cand = bv > -2.0;            // some row >= k exists (always, as k < m)
+      const bool pwait = cand && brw >= 0.0;  // pivot row not escaped: poll its data
+      const bool owait = rk_key == 1.0;       // old row k not escaped
+      double pd = 0.0, od = 0.0;
+      if (lane <= W) {
+        if (cand) pd = ld_agent(&xaj[(long)bq * SW + 2 + lane]);
+        od = ld_agent(&xaj[(long)LEAF_GMAX * SW + lane]);
+      }
+      for (;;) {
+        const bool pend = lane <= W && ((pwait && is_xsent(pd)) || (owait && is_xsent(od)));
+        if (__ballot(pend) == 0) break;
+        if (++spins > SPIN_LIMIT) {
+          if (lane == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (lane <= W && pwait && is_xsent(pd)) pd = ld_agent(&xaj[(long)bq * SW + 2 + lane]);
+        if (lane <= W && owait && is_xsent(od)) od = ld_agent(&xaj[(long)LEAF_GMAX * SW + lane]);
+      }
+      if (lane <= W) {
+        xs[lane] = pd;
+        xs[W + 1 + lane] = od;
+      }
+      if (lane == 0) {
+        const int br = (int)(brw < 0.0 ? -brw - 1.0 : brw);
+        pinfo = cand ? br : k;
+      }
+      // every workgroup has published this column, so all of them are done
+      // reading the previous one: its area gets the sentinel back
+      const double sv = __longlong_as_double((long long)XSENT);
+      if (lane < SW) st_agent(&xareset[(long)g * SW + lane], sv);
+      if (g == 0 && lane < W + 2) st_agent(&xareset[(long)LEAF_GMAX * SW + lane], sv);
     }
     __syncthreads();
-    // 4. the pivot (every wave agrees on its own): the first workgroup
-    // holding the largest key (workgroups are in row order)
-    double bv;
-    const int bq = wave_first_max(lane < G ? xs[lane * SW] : -3.0, bv);
-    const int br = (int)xs[bq * SW + 1];
-    const int p = bv > -2.0 ? br : k;
+    // 4. the pivot, the pivot row and the old row k, for every thread
+    const int p = pinfo;
     if (t == 0) {
       pv[j] = p;
       if (g == 0) a.piv[k] = p;
     }
-    const double* prow = xs + bq * SW + 2;  // the pivot row; prow[W] = its perm
-    const double* orow = xs + G * SW;       // the old row k
+    const double* prow = xs;          // the pivot row; prow[W] = its perm
+    const double* orow = xs + W + 1;  // the old row k
     // 5. interchange in registers
     if (p != k && (row == p || row == k)) {
       const double* src = row == p ? orow : prow;
@@ -273,24 +358,19 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
     // 6. column of L and rank-1 update of the rows below k (columns j+1 ..
     // w-1 are x[1 .. w-1-j]; the rest hold finished columns or padding)
     if (mine && row > k) {
-      const double l = x[0] * (1.0 / prow[0]);
+      // the whole pivot row into registers first (every LDS read in flight
+      // together), then the divide and the updates
+      double p[W];
+#pragma unroll
+      for (int c = 0; c < W; ++c) p[c] = prow[c];
+      const double l = x[0] * (1.0 / p[0]);
       x[0] = l;
       const int nc = __builtin_amdgcn_readfirstlane(w - 1 - j);
-      // eight columns at a time: the next group's LDS reads wait for this
-      // group's last result, so at most a few rows of prow sit in registers
-      typedef __attribute__((address_space(3))) const double lds_double;
-      lds_double* pr = (lds_double*)prow;
 #pragma unroll
-      for (int c8 = 0; c8 < W; c8 += 8) {
-        asm volatile("" : "+v"(pr) : "v"(x[c8 == 0 ? 0 : c8 - 1]));
-#pragma unroll
-        for (int c = c8; c < c8 + 8; ++c) {
-          if (c == 0) continue;
-          // computed for every c, kept only for the active columns: no
-          // branch, so the LDS reads of a group are issued together
-          const double nv = fma(-l, pr[c], x[c]);
-          x[c] = c <= nc ? nv : x[c];
-        }
+      for (int c = 1; c < W; ++c) {
+        // computed for every c, kept only for the active columns (no branch)
+        const double nv = fma(-l, p[c], x[c]);
+        x[c] = c <= nc ? nv : x[c];
       }
     }
     // 7. rotate: column j moves to the back
@@ -402,6 +482,9 @@ __global__ void k_inv_perm(const int* perm, int k, int m, int* inv0) {
     inv0[perm[r]] = r;
 }
 __global__ void k_zero1(int* p) { p[0] = 0; }
+__global__ void k_fill_u64_lu(unsigned long long* p, long n, unsigned long long v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
 __global__ void k_perm_pairs(const int* perm, const int* inv0, int k, int m, int* pairs,
                              int maxpairs) {
   for (int r = k + blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
@@ -500,12 +583,12 @@ struct LuCtx {
   int* piv;
   int* perm;
   int* inv0;
-  int* pairs[2];
+  int* pairs[LU_NPAR];
   int maxpairs;
-  double* linv[2];  // inverse of the unit upper T11 of a panel (nb x nb)
+  double* linv[LU_NPAR];  // inverse of the unit upper T11 of a panel (nb x nb)
   double* tmp[2];   // U12 staging for the two streams
-  double* xb;
-  double* rowk;
+  double* xa;          // leaf exchange records (ring of LEAF_RING areas)
+  unsigned col_count;  // leaf columns factored so far (ring position)
   double* stage;
   unsigned* ctr;
   unsigned sync_base;  // barrier arrivals queued so far on ctr
@@ -533,8 +616,8 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   a.pc1 = pc1;
   a.piv = c.piv;
   a.perm = c.perm;
-  a.xb = c.xb;
-  a.rowk = c.rowk;
+  a.xa = c.xa;
+  a.ring0 = (int)(c.col_count % LEAF_RING);
   a.stage = c.stage;
   a.ctr = c.ctr;
   a.base = c.sync_base;
@@ -542,9 +625,10 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   void* args[] = {&a};
   MOPS_CHECK_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_leaf, dim3(G), dim3(LEAF_R),
                                             args, 0, s));
-  // one barrier per column, plus one for the gather when the panel is wider
-  const int nsync = w + ((pc1 - pc0) - w > 0 ? 1 : 0);
+  // one grid barrier, for the gather, when the panel is wider than the leaf
+  const int nsync = (pc1 - pc0) - w > 0 ? 1 : 0;
   c.sync_base += (unsigned)nsync * (unsigned)G;
+  c.col_count += (unsigned)w;
   return ST_OK;
 }
 
@@ -659,11 +743,11 @@ size_t getrf_ws_bytes(int m, int n, int nb) {
   if (nb > mn) nb = mn;
   const int G = ceil_div(m, LEAF_R);
   size_t b = (size_t)m * n * sizeof(double);                   // T
-  b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;   // exchange
+  b += (size_t)LEAF_RING * LEAF_XAREA * sizeof(double) + 4096;   // exchange
   b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
   b += (size_t)(mn + 3 * m + 64) * sizeof(int);                // piv, perm, inv0, ctr, err
-  b += (size_t)2 * (4 * nb + 2) * sizeof(int);                 // pairs
-  b += (size_t)2 * nb * nb * sizeof(double);                   // linv
+  b += (size_t)LU_NPAR * (4 * nb + 2) * sizeof(int);           // pairs
+  b += (size_t)LU_NPAR * nb * nb * sizeof(double);             // linv
   b += (size_t)2 * n * nb * sizeof(double);                    // tmp
   b += trtri_ws_bytes(nb) + trsm_ws_bytes(F_RIGHT, nb, nb) + 32 * 256;
   return b;
@@ -689,27 +773,31 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   c.ldT = n;
   c.maxpairs = 2 * nb;
   c.T = (double*)ws_carve((size_t)m * n * sizeof(double));
-  c.xb = (double*)ws_carve((size_t)2 * G * (LEAF_W + 3) * sizeof(double));
-  c.rowk = (double*)ws_carve(2 * (LEAF_W + 1) * sizeof(double));
+  c.xa = (double*)ws_carve((size_t)LEAF_RING * LEAF_XAREA * sizeof(double));
+  c.col_count = 0;
   c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
   c.piv = (int*)ws_carve((size_t)mn * sizeof(int));
   c.perm = (int*)ws_carve((size_t)m * sizeof(int));
   c.inv0 = (int*)ws_carve((size_t)m * sizeof(int));
-  c.pairs[0] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
-  c.pairs[1] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
-  c.linv[0] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
-  c.linv[1] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
+  bool slots_ok = true;
+  for (int q = 0; q < LU_NPAR; ++q) {
+    c.pairs[q] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
+    c.linv[q] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
+    slots_ok = slots_ok && c.pairs[q] && c.linv[q];
+  }
   c.tmp[0] = (double*)ws_carve((size_t)n * nb * sizeof(double));
   c.tmp[1] = (double*)ws_carve((size_t)n * nb * sizeof(double));
   c.ctr = (unsigned*)ws_carve(256);
   c.err = err_flag ? err_flag : (int*)ws_carve(256);
-  if (!c.T || !c.xb || !c.rowk || !c.stage || !c.piv || !c.perm || !c.inv0 || !c.pairs[0] || !c.pairs[1] ||
-      !c.linv[0] || !c.linv[1] || !c.tmp[0] || !c.tmp[1] || !c.ctr || !c.err) {
+  if (!c.T || !c.xa || !c.stage || !c.piv || !c.perm || !c.inv0 || !slots_ok ||
+      !c.tmp[0] || !c.tmp[1] || !c.ctr || !c.err) {
     set_error("getrf: workspace too small");
     return ST_NOMEM;
   }
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.err, 1, 0);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
+  hipLaunchKernelGGL(k_fill_u64_lu, dim3(grid1(LEAF_RING * LEAF_XAREA)), dim3(256), 0, s,
+                     (unsigned long long*)c.xa, (long)LEAF_RING * LEAF_XAREA, XSENT);
   c.sync_base = 0;
   hipLaunchKernelGGL(k_iota, dim3(grid1(m)), dim3(256), 0, s, c.perm, m);
   // T = A^T
@@ -740,21 +828,28 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
   bool have_rest = false;
   int par = 0;
-  for (int k = 0; k < mn; k += nb, par ^= 1) {
+  for (int k = 0; k < mn; k += nb, par = (par + 1) % LU_NPAR) {
     const int kb = min(nb, mn - k);
     const int kb1 = min(nb, mn - k - kb);  // width of the next panel (0 at the end)
     const int j_next = k + kb, j_rest = k + kb + kb1;
+    // s2 brings block column k+1 up to date only after step k-1 has
     if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
     MOPS_CHECK_HIP(hipStreamWaitEvent(sr, e_pan, 0));
-    MOPS_TRY(apply_rows(sr, c, par, 0, k));
+    have_rest = false;
     if (j_rest < n) {
-      MOPS_TRY(update_cols(sr, c, k, kb, par, j_rest, n, c.tmp[0], true));
+      // block column k+2 first: the next step's block column update needs
+      // only it, so the panel chain does not wait for the whole rest
+      const int j_far = min(n, j_rest + nb);
+      MOPS_TRY(update_cols(sr, c, k, kb, par, j_rest, j_far, c.tmp[0], true));
       MOPS_CHECK_HIP(hipEventRecord(e_rest, sr));
       have_rest = true;
+      MOPS_TRY(update_cols(sr, c, k, kb, par, j_far, n, c.tmp[0] + (long)(j_far - j_rest) * kb,
+                           true));
     }
+    MOPS_TRY(apply_rows(sr, c, par, 0, k));
     if (kb1 > 0) {
       MOPS_TRY(update_cols(s2, c, k, kb, par, j_next, j_next + kb1, c.tmp[1], false));
-      MOPS_TRY(panel(s2, c, j_next, kb1, par ^ 1));
+      MOPS_TRY(panel(s2, c, j_next, kb1, (par + 1) % LU_NPAR));
       MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     }
   }
@@ -790,7 +885,7 @@ namespace {
 
 struct MixedCtx {
   float* T32;        // n x n, T32[c + r*n] = A[r][c]
-  float* linv32[2];  // FP32 copies of the panels' unit-triangle inverses
+  float* linv32[LU_NPAR];  // FP32 copies of the panels' unit-triangle inverses
   float* tmp32[2];   // U12 staging for the two streams
   double* P64;       // one FP64 panel: nb x n, P64[(c-k) + (r-k)*nb]
 };
@@ -835,6 +930,7 @@ int panel_mixed(hipStream_t s, LuCtx& c, MixedCtx& x, int k, int kb, int par) {
   pc.ldT = kb;
   MOPS_TRY(panel(s, pc, k, kb, par));
   c.sync_base = pc.sync_base;
+  c.col_count = pc.col_count;
   MOPS_TRY(convert_d2s(s, kb, kb, c.linv[par], kb, x.linv32[par], kb));
   return convert_d2s(s, kb, rows, x.P64, kb, x.T32 + k + (long)k * ld, ld);
 }
@@ -848,11 +944,11 @@ size_t getrf_mixed_ws_bytes(int n, int nb) {
   const int G = ceil_div(n, LEAF_R);
   size_t b = (size_t)n * n * sizeof(float);                    // T32
   b += (size_t)nb * n * sizeof(double);                        // P64
-  b += (size_t)2 * G * (LEAF_W + 3) * sizeof(double) + 4096;   // exchange
+  b += (size_t)LEAF_RING * LEAF_XAREA * sizeof(double) + 4096;   // exchange
   b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
   b += (size_t)(4 * n + 64) * sizeof(int);                     // piv, perm, inv0, ctr, err
-  b += (size_t)2 * (4 * nb + 2) * sizeof(int);                 // pairs
-  b += (size_t)2 * nb * nb * (sizeof(double) + sizeof(float));  // linv
+  b += (size_t)LU_NPAR * (4 * nb + 2) * sizeof(int);           // pairs
+  b += (size_t)LU_NPAR * nb * nb * (sizeof(double) + sizeof(float));  // linv
   b += (size_t)2 * n * nb * sizeof(float);                     // tmp32
   b += trtri_ws_bytes(nb) + 32 * 256;
   return b;
@@ -877,29 +973,34 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   c.maxpairs = 2 * nb;
   x.T32 = (float*)ws_carve((size_t)n * n * sizeof(float));
   x.P64 = (double*)ws_carve((size_t)nb * n * sizeof(double));
-  c.xb = (double*)ws_carve((size_t)2 * G * (LEAF_W + 3) * sizeof(double));
-  c.rowk = (double*)ws_carve(2 * (LEAF_W + 1) * sizeof(double));
+  c.xa = (double*)ws_carve((size_t)LEAF_RING * LEAF_XAREA * sizeof(double));
+  c.col_count = 0;
   c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
   c.piv = (int*)ws_carve((size_t)n * sizeof(int));
   c.perm = (int*)ws_carve((size_t)n * sizeof(int));
   c.inv0 = (int*)ws_carve((size_t)n * sizeof(int));
-  for (int q = 0; q < 2; ++q) {
+  bool slots_ok = true;
+  for (int q = 0; q < LU_NPAR; ++q) {
     c.pairs[q] = (int*)ws_carve((size_t)(2 * c.maxpairs + 1) * sizeof(int));
     c.linv[q] = (double*)ws_carve((size_t)nb * nb * sizeof(double));
     x.linv32[q] = (float*)ws_carve((size_t)nb * nb * sizeof(float));
+    slots_ok = slots_ok && c.pairs[q] && c.linv[q] && x.linv32[q];
+  }
+  for (int q = 0; q < 2; ++q) {
     x.tmp32[q] = (float*)ws_carve((size_t)n * nb * sizeof(float));
     c.tmp[q] = nullptr;
   }
   c.ctr = (unsigned*)ws_carve(256);
   c.err = err_flag ? err_flag : (int*)ws_carve(256);
-  if (!x.T32 || !x.P64 || !c.xb || !c.rowk || !c.stage || !c.piv || !c.perm || !c.inv0 ||
-      !c.pairs[0] || !c.pairs[1] || !c.linv[0] || !c.linv[1] || !x.linv32[0] || !x.linv32[1] ||
-      !x.tmp32[0] || !x.tmp32[1] || !c.ctr || !c.err) {
+  if (!x.T32 || !x.P64 || !c.xa || !c.stage || !c.piv || !c.perm || !c.inv0 ||
+      !slots_ok || !x.tmp32[0] || !x.tmp32[1] || !c.ctr || !c.err) {
     set_error("getrf_mixed: workspace too small");
     return ST_NOMEM;
   }
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c.err, 1, 0);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, c.ctr);
+  hipLaunchKernelGGL(k_fill_u64_lu, dim3(grid1(LEAF_RING * LEAF_XAREA)), dim3(256), 0, s,
+                     (unsigned long long*)c.xa, (long)LEAF_RING * LEAF_XAREA, XSENT);
   c.sync_base = 0;
   hipLaunchKernelGGL(k_iota, dim3(grid1(n)), dim3(256), 0, s, c.perm, n);
   hipLaunchKernelGGL((k_transpose<double, float>), dim3(ceil_div(n, 64), ceil_div(n, 64)),
@@ -922,21 +1023,26 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
   bool have_rest = false;
   int par = 0;
-  for (int k = 0; k < n; k += nb, par ^= 1) {
+  for (int k = 0; k < n; k += nb, par = (par + 1) % LU_NPAR) {
     const int kb = min(nb, n - k);
     const int kb1 = min(nb, n - k - kb);
     const int j_next = k + kb, j_rest = k + kb + kb1;
     if (have_rest) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_rest, 0));
     MOPS_CHECK_HIP(hipStreamWaitEvent(sr, e_pan, 0));
-    MOPS_TRY(apply_rows32(sr, c, x, par, 0, k));
+    have_rest = false;
     if (j_rest < n) {
-      MOPS_TRY(update_cols32(sr, c, x, k, kb, par, j_rest, n, x.tmp32[0], true));
+      // block column k+2 first (see getrf_d)
+      const int j_far = min(n, j_rest + nb);
+      MOPS_TRY(update_cols32(sr, c, x, k, kb, par, j_rest, j_far, x.tmp32[0], true));
       MOPS_CHECK_HIP(hipEventRecord(e_rest, sr));
       have_rest = true;
+      MOPS_TRY(update_cols32(sr, c, x, k, kb, par, j_far, n,
+                             x.tmp32[0] + (long)(j_far - j_rest) * kb, true));
     }
+    MOPS_TRY(apply_rows32(sr, c, x, par, 0, k));
     if (kb1 > 0) {
       MOPS_TRY(update_cols32(s2, c, x, k, kb, par, j_next, j_next + kb1, x.tmp32[1], false));
-      MOPS_TRY(panel_mixed(s2, c, x, j_next, kb1, par ^ 1));
+      MOPS_TRY(panel_mixed(s2, c, x, j_next, kb1, (par + 1) % LU_NPAR));
       MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));
     }
   }
