@@ -67,6 +67,7 @@ int mops_d_dgemm(int flags, int M, int N, int P, double alpha, const double* A, 
   if (lda < (arows > 1 ? arows : 1)) return -7;
   if (ldb < (brows > 1 ? brows : 1)) return -9;
   if (ldc < (M > 1 ? M : 1)) return -12;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(gemm_split_ws_bytes(M, N, P, sizeof(double))));
   return gemm_d(as_stream(stream), ta, tb, M, N, P, alpha, A, lda, B, ldb, beta, C, ldc);
 }
@@ -91,6 +92,7 @@ int mops_dmult_mm_blocked3(mops_mdata_t* C, const mops_mdata_t* A, const mops_md
     return ST_HIP;
   }
   const size_t na = (size_t)rows * P, nbb = (size_t)P * cols, nc = (size_t)rows * cols;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + nbb + nc) * sizeof(double) + 1024 +
                       gemm_split_ws_bytes(rows, cols, P, sizeof(double))));
   double* dA = (double*)ws_carve(na * sizeof(double) + 8);
@@ -125,6 +127,7 @@ int mops_d_dtrsm(int flags, int N, int nrhs, double alpha, const double* A, int 
   const int brows = (flags & F_RIGHT) ? nrhs : N;
   if (ldb < (brows > 1 ? brows : 1)) return -8;
   if (N == 0 || nrhs == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(trsm_ws_bytes(flags, N, nrhs)));
   return trsm_d(as_stream(stream), flags, N, nrhs, alpha, A, lda, B, ldb);
 }
@@ -139,6 +142,7 @@ int mops_d_dsyrk(int flags, int N, int P, double alpha, const double* A, int lda
   if (lda < (arows > 1 ? arows : 1)) return -6;
   if (ldc < (N > 1 ? N : 1)) return -9;
   const int mode = (flags & F_LOWER) ? GM_LOWER : GM_UPPER;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(gemm_split_ws_bytes(N, N, P, sizeof(double))));
   return gemm_d(as_stream(stream), ta, !ta, N, N, P, alpha, A, lda, A, lda, beta, C, ldc, mode);
 }
@@ -158,6 +162,7 @@ int mops_dmmat_solve_blk(mops_mdata_t* B, const mops_mdata_t* A, double alpha, i
     return ST_HIP;
   }
   const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + trsm_ws_bytes(flags, N, nrhs) + 1024));
   double* dA = (double*)ws_carve(na * sizeof(double));
   double* dB = (double*)ws_carve(nb * sizeof(double));
@@ -187,6 +192,7 @@ int mops_dmmat_rank_blk(mops_mdata_t* C, const mops_mdata_t* A, double alpha, do
     return ST_HIP;
   }
   const size_t na = (size_t)n * P, nc = (size_t)n * n;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + nc) * sizeof(double) + 1024 +
                       gemm_split_ws_bytes(n, n, P, sizeof(double))));
   double* dA = (double*)ws_carve(na * sizeof(double));

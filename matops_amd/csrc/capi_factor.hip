@@ -42,6 +42,7 @@ int mops_d_dpotrf(int flags, int N, int nb, double* A, int lda, int* info, void*
   if (nb < 0) return -3;
   if (lda < (N > 1 ? N : 1)) return -5;
   if (N == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(potrf_ws_bytes(N, nb)));
   return potrf_d(as_stream(stream), flags, N, nb, A, lda, info, nullptr);
 }
@@ -54,6 +55,7 @@ int mops_d_dpotrs(int flags, int N, int nrhs, const double* A, int lda, double* 
   if (lda < (N > 1 ? N : 1)) return -5;
   if (ldb < (N > 1 ? N : 1)) return -7;
   if (N == 0 || nrhs == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(potrs_ws_bytes(flags, N, nrhs)));
   return potrs_d(as_stream(stream), flags, N, nrhs, A, lda, B, ldb);
 }
@@ -68,6 +70,7 @@ int mops_dpotrf(mops_mdata_t* A, int flags, int N, int nb) {
   hipStream_t s = need_stream();
   if (!s) return ST_HIP;
   const size_t na = (size_t)N * N;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve(na * sizeof(double) + potrf_ws_bytes(N, nb) + 2048));
   double* dA = (double*)ws_carve(na * sizeof(double));
   int* dinfo = (int*)ws_carve(sizeof(int));
@@ -100,6 +103,7 @@ int mops_d_dpotrf_adjoint(int N, int nb, const double* L, int ldl, double* Abar,
   if (ldl < (N > 1 ? N : 1)) return -4;
   if (lda < (N > 1 ? N : 1)) return -6;
   if (N == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(chol_adjoint_ws_bytes(N, nb)));
   return chol_adjoint_d(as_stream(stream), N, nb, L, ldl, Abar, lda);
 }
@@ -116,6 +120,7 @@ int mops_dpotrf_adjoint(const mops_mdata_t* L, mops_mdata_t* Lbar, int N, int nb
   hipStream_t s = need_stream();
   if (!s) return ST_HIP;
   const size_t na = (size_t)N * N;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve(2 * na * sizeof(double) + chol_adjoint_ws_bytes(N, nb) + 2048));
   double* dL = (double*)ws_carve(na * sizeof(double));
   double* dB = (double*)ws_carve(na * sizeof(double));
@@ -137,6 +142,7 @@ int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nr
   hipStream_t s = need_stream();
   if (!s) return ST_HIP;
   const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + potrs_ws_bytes(flags, N, nrhs) + 1024));
   double* dA = (double*)ws_carve(na * sizeof(double));
   double* dB = (double*)ws_carve(nb * sizeof(double));
@@ -165,6 +171,7 @@ int mops_d_dgetrf(int M, int N, int nb, double* A, int lda, int64_t* ipiv, int* 
   if (nb < 0) return -3;
   if (lda < (M > 1 ? M : 1)) return -5;
   if (M == 0 || N == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(getrf_ws_bytes(M, N, nb)));
   return getrf_d(as_stream(stream), M, N, nb, A, lda, reinterpret_cast<long long*>(ipiv), perm,
                  status);
@@ -179,6 +186,7 @@ int mops_d_dgetrs(int flags, int N, int nrhs, const double* A, int lda, const in
   if (!perm && !ipiv) return -6;
   if (ldb < (N > 1 ? N : 1)) return -10;
   if (N == 0 || nrhs == 0) return ST_OK;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(getrs_ws_bytes(N, nrhs)));
   return getrs_d(as_stream(stream), flags, N, nrhs, A, lda,
                  reinterpret_cast<const long long*>(ipiv), npiv, perm, B, ldb);
@@ -201,6 +209,7 @@ int mops_dgetrf(mops_mdata_t* A, int64_t* pivots, int npiv, int M, int N, int nb
   hipStream_t s = need_stream();
   if (!s) return ST_HIP;
   const size_t na = (size_t)M * N;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve(na * sizeof(double) + (size_t)mn * 8 + getrf_ws_bytes(M, N, nb) + 4096));
   double* dA = (double*)ws_carve(na * sizeof(double));
   long long* dp = (long long*)ws_carve((size_t)mn * sizeof(long long));
@@ -232,6 +241,7 @@ int mops_dgetrs(mops_mdata_t* B, const mops_mdata_t* A, const int64_t* pivots, i
   if (!s) return ST_HIP;
   const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
   const int np = npiv < N ? npiv : N;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + (size_t)np * 8 + getrs_ws_bytes(N, nrhs) +
                       4096));
   double* dA = (double*)ws_carve(na * sizeof(double));
@@ -268,6 +278,7 @@ int mops_d_dgesv_ir(int N, int nrhs, const double* A, int lda, const double* B, 
     return -9;
   }
   if (!rep) return -12;
+  WsUse wu(as_stream(stream));
   MOPS_TRY(ws_reserve(gesv_ir_ws_bytes(N, nrhs, prec)));
   return gesv_ir_d(as_stream(stream), N, nrhs, A, lda, B, ldb, X, ldx, prec, tol, maxit, rep);
 }
@@ -293,6 +304,7 @@ int mops_dgesv_ir(const mops_mdata_t* A, const mops_mdata_t* B, mops_mdata_t* X,
   hipStream_t s = need_stream();
   if (!s) return ST_HIP;
   const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  WsUse wu(s);
   MOPS_TRY(ws_reserve((na + 2 * nb) * sizeof(double) + gesv_ir_ws_bytes(N, nrhs, prec) + 4096));
   double* dA = (double*)ws_carve(na * sizeof(double));
   double* dB = (double*)ws_carve(nb * sizeof(double));

@@ -81,6 +81,22 @@ void ws_release(size_t mark);
 // Frees the calling thread's workspace and streams on every device.
 void thread_release();
 
+// Held by every entry point for as long as it queues work that uses the
+// workspace on stream s. The workspace is reused by the next call of the
+// same thread, which may name another stream: the constructor makes s wait
+// for the work the previous call queued (when that was on another stream),
+// the destructor marks the end of this call's work on s.
+class WsUse {
+ public:
+  explicit WsUse(hipStream_t s);
+  ~WsUse();
+  WsUse(const WsUse&) = delete;
+  WsUse& operator=(const WsUse&) = delete;
+
+ private:
+  hipStream_t s_;
+};
+
 // Per-thread stream used by the host-pointer entry points.
 hipStream_t host_stream();
 // Per-thread second stream (default priority) for look-ahead work, and six

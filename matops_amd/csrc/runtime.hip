@@ -24,6 +24,9 @@ struct ThreadState {
   hipStream_t bulk[16] = {nullptr};
   bool bulk_tried[16] = {};
   hipEvent_t ev[16][6] = {};
+  hipEvent_t ws_done[16] = {};          // end of the last call's work
+  hipStream_t ws_done_stream[16] = {};  // the stream it was recorded on
+  bool ws_done_set[16] = {};
 };
 thread_local ThreadState g_ts;
 
@@ -74,6 +77,35 @@ int ws_reserve(size_t bytes) {
   return ST_OK;
 }
 
+WsUse::WsUse(hipStream_t s) : s_(s) {
+  const int d = current_device();
+  if (g_ts.ws_done_set[d] && g_ts.ws_done_stream[d] != s &&
+      hipStreamWaitEvent(s, g_ts.ws_done[d], 0) != hipSuccess) {
+    // fall back to waiting on the host: never overlap two calls' workspace
+    (void)hipGetLastError();
+    (void)hipEventSynchronize(g_ts.ws_done[d]);
+  }
+}
+
+WsUse::~WsUse() {
+  const int d = current_device();
+  if (!g_ts.ws_done[d] &&
+      hipEventCreateWithFlags(&g_ts.ws_done[d], hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    g_ts.ws_done[d] = nullptr;
+    g_ts.ws_done_set[d] = false;
+    return;
+  }
+  if (hipEventRecord(g_ts.ws_done[d], s_) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(s_);
+    g_ts.ws_done_set[d] = false;
+    return;
+  }
+  g_ts.ws_done_stream[d] = s_;
+  g_ts.ws_done_set[d] = true;
+}
+
 void* ws_carve(size_t bytes) {
   Workspace& w = g_ts.ws[current_device()];
   size_t off = align_up(w.used, 256);
@@ -99,6 +131,8 @@ void thread_release() {
     if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d]) continue;
     if (hipSetDevice(d) != hipSuccess) continue;
     if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
+    // the last call's work (on whatever stream) is done with the workspace
+    if (g_ts.ws_done_set[d]) (void)hipEventSynchronize(g_ts.ws_done[d]);
     if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
     if (g_ts.stream[d]) (void)hipStreamDestroy(g_ts.stream[d]);
     if (g_ts.aux[d]) {
@@ -111,6 +145,10 @@ void thread_release() {
     }
     for (int e = 0; e < 6; ++e)
       if (g_ts.ev[d][e]) (void)hipEventDestroy(g_ts.ev[d][e]);
+    if (g_ts.ws_done[d]) (void)hipEventDestroy(g_ts.ws_done[d]);
+    g_ts.ws_done[d] = nullptr;
+    g_ts.ws_done_set[d] = false;
+    g_ts.ws_done_stream[d] = nullptr;
     g_ts.ws[d] = Workspace();
     g_ts.stream[d] = nullptr;
     g_ts.aux[d] = nullptr;

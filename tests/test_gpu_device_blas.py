@@ -108,3 +108,29 @@ def test_d_dtrsm(flags, n, nrhs):
     R = (X @ Top - 2.0 * B0) if right else (Top @ X - 2.0 * B0)
     scale = (np.abs(X) @ np.abs(Top)) if right else (np.abs(Top) @ np.abs(X))
     assert np.all(np.abs(R) <= 4 * n * EPS * (scale + 2.0 * np.abs(B0)) + 1e-300)
+
+
+def test_calls_on_two_streams_share_the_workspace():
+    """Two GEMMs with few output tiles and a long k (their partial tiles
+    live in the calling thread's workspace) queued back to back on two
+    streams from one host thread, three times over: each call waits for
+    the previous call's use of the workspace, so both results are right."""
+    import torch
+    from matops_amd import _lib
+    L = _lib.lib()
+    m, n, k = 64, 128, 6000
+    A1, B1, A2, B2 = fmat(m, k, 71), fmat(k, n, 72), fmat(m, k, 73), fmat(k, n, 74)
+    dA1, dB1, dA2, dB2 = _dev(A1), _dev(B1), _dev(A2), _dev(B2)
+    C1 = torch.zeros((n, m), dtype=torch.float64, device="cuda:0").T
+    C2 = torch.zeros((n, m), dtype=torch.float64, device="cuda:0").T
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        assert L.mops_d_dgemm(0, m, n, k, 1.0, dA1.data_ptr(), m, dB1.data_ptr(), k, 0.0,
+                              C1.data_ptr(), m, ctypes.c_void_p(s1.cuda_stream)) == 0
+        assert L.mops_d_dgemm(0, m, n, k, 1.0, dA2.data_ptr(), m, dB2.data_ptr(), k, 0.0,
+                              C2.data_ptr(), m, ctypes.c_void_p(s2.cuda_stream)) == 0
+    torch.cuda.synchronize()
+    for C, A, B in ((C1, A1, B1), (C2, A2, B2)):
+        Ch = np.asfortranarray(C.cpu().numpy())
+        assert np.all(np.abs(Ch - A @ B) <= gemm_bound(np.abs(A), np.abs(B), k))
