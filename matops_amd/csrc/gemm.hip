@@ -106,6 +106,9 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
 
   static_assert(BK % (4 * KS) == 0, "k split");
+  // 128x128 tiles keep two slabs in flight (one WG per CU: the latency of a
+  // slab's loads needs more than one slab of MFMAs to hide)
+  constexpr bool PF2 = BM == 128 && BN == 128 && KS == 1;
   constexpr int SMEM_AB = 2 * BK * (LDS_A + LDS_B);
   constexpr int SMEM_RED = (KS - 1) * BM * BN;
   __shared__ T smem[SMEM_AB > SMEM_RED ? SMEM_AB : SMEM_RED];
@@ -216,31 +219,31 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     }
     return v;
   };
-  auto load = [&](int k0) {
+  auto load_to = [&](int k0, pair_t(&xa)[A_PP], pair_t(&xb)[B_PP]) {
     const bool inner = inner_mn && k0 + BK <= K;
 #pragma unroll
     for (int e = 0; e < A_PP; ++e) {
       int i, kk;
       coords(e, BM, !TA, i, kk);
-      ra[e] = fetch(A, lda, !TA, m0, M, i, k0 + kk, inner);
+      xa[e] = fetch(A, lda, !TA, m0, M, i, k0 + kk, inner);
     }
 #pragma unroll
     for (int e = 0; e < B_PP; ++e) {
       int j, kk;
       coords(e, BN, TB, j, kk);
-      rb[e] = fetch(B, ldb, TB, n0, N, j, k0 + kk, inner);
+      xb[e] = fetch(B, ldb, TB, n0, N, j, k0 + kk, inner);
     }
   };
-  auto store = [&](int buf) {
+  auto store_from = [&](int buf, const pair_t(&xa)[A_PP], const pair_t(&xb)[B_PP]) {
 #pragma unroll
     for (int e = 0; e < A_PP; ++e) {
       int i, kk;
       coords(e, BM, !TA, i, kk);
       if (!TA) {
-        *(pair_t*)&As[buf][kk * LDS_A + i] = ra[e];
+        *(pair_t*)&As[buf][kk * LDS_A + i] = xa[e];
       } else {
-        As[buf][kk * LDS_A + i] = ra[e].x;
-        As[buf][(kk + 1) * LDS_A + i] = ra[e].y;
+        As[buf][kk * LDS_A + i] = xa[e].x;
+        As[buf][(kk + 1) * LDS_A + i] = xa[e].y;
       }
     }
 #pragma unroll
@@ -248,15 +251,56 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
       int j, kk;
       coords(e, BN, TB, j, kk);
       if (TB) {
-        *(pair_t*)&Bs[buf][kk * LDS_B + j] = rb[e];
+        *(pair_t*)&Bs[buf][kk * LDS_B + j] = xb[e];
       } else {
-        Bs[buf][kk * LDS_B + j] = rb[e].x;
-        Bs[buf][(kk + 1) * LDS_B + j] = rb[e].y;
+        Bs[buf][kk * LDS_B + j] = xb[e].x;
+        Bs[buf][(kk + 1) * LDS_B + j] = xb[e].y;
       }
     }
   };
+  auto load = [&](int k0) { load_to(k0, ra, rb); };
+  auto store = [&](int buf) { store_from(buf, ra, rb); };
+  // the MFMAs of one slab (this k share's k steps) from LDS buffer buf
+  auto compute = [&](int buf) {
+    const T* as = &As[buf][0];
+    const T* bs = &Bs[buf][0];
+#pragma unroll
+    for (int kk = kh * (BK / KS); kk < (kh + 1) * (BK / KS); kk += 4) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + lk) * LDS_A + wm * WM + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + lk) * LDS_B + wn * WN + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
+    }
+  };
 
-  if (kbeg < kend) {
+  if constexpr (PF2) {
+    // two slabs in flight: slab s+2 is fetched into one register set while
+    // the MFMAs run on slab s (LDS) and the other set (slab s+1) goes to
+    // the other LDS buffer
+    pair_t ra2[A_PP], rb2[B_PP];
+    if (kbeg < kend) {
+      load_to(kbeg, ra, rb);
+      store_from(0, ra, rb);
+      if (kbeg + BK < kend) load_to(kbeg + BK, ra2, rb2);
+      __syncthreads();
+      for (int k0 = kbeg; k0 < kend; k0 += 2 * BK) {
+        if (k0 + 2 * BK < kend) load_to(k0 + 2 * BK, ra, rb);
+        compute(0);
+        if (k0 + BK < kend) store_from(1, ra2, rb2);
+        __syncthreads();
+        if (k0 + BK >= kend) break;
+        if (k0 + 3 * BK < kend) load_to(k0 + 3 * BK, ra2, rb2);
+        compute(1);
+        if (k0 + 2 * BK < kend) store_from(0, ra, rb);
+        __syncthreads();
+      }
+    }
+  } else if (kbeg < kend) {
     load(kbeg);
     store(0);
     __syncthreads();
@@ -264,20 +308,7 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
       const bool more = k0 + BK < kend;
       if (more) load(k0 + BK);
-      const T* as = &As[buf][0];
-      const T* bs = &Bs[buf][0];
-#pragma unroll
-      for (int kk = kh * (BK / KS); kk < (kh + 1) * (BK / KS); kk += 4) {
-        T af[FM], bf[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = as[(kk + lk) * LDS_A + wm * WM + i * 16 + lr];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + lk) * LDS_B + wn * WN + j * 16 + lr];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
-      }
+      compute(buf);
       if (more) store(buf ^ 1);
       __syncthreads();
       buf ^= 1;
