@@ -106,9 +106,12 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
 
   static_assert(BK % (4 * KS) == 0, "k split");
-  // 128x128 tiles keep two slabs in flight (one WG per CU: the latency of a
-  // slab's loads needs more than one slab of MFMAs to hide)
-  constexpr bool PF2 = BM == 128 && BN == 128 && KS == 1;
+  // 128x128 tiles and the instances with two k shares keep two slabs in
+  // flight: with 8 waves per workgroup they run few workgroups per CU, and
+  // a slab's loads need more than one slab of MFMAs to arrive (DGEMM 1024^3
+  // 68 -> 65 us). The 4-wave triangle instance does not: more registers
+  // made the SYRK of the Cholesky slower.
+  constexpr bool PF2 = (BM == 128 && BN == 128 && KS == 1) || KS == 2;
   constexpr int SMEM_AB = 2 * BK * (LDS_A + LDS_B);
   constexpr int SMEM_RED = (KS - 1) * BM * BN;
   __shared__ T smem[SMEM_AB > SMEM_RED ? SMEM_AB : SMEM_RED];
