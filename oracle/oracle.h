@@ -10,7 +10,7 @@
  *    the level-1/2 helpers the unblocked panels call. They are reached
  *    through a table, so the same glue runs either on this file set's own
  *    restatement ("port", calgo_port.c) or on the reference's calgo C
- *    compiled from /root/reference into oracle/_ref/ ("reference").
+ *    compiled by oracle/Makefile into oracle/_ref/ ("reference").
  *  - the Go glue restated in C line by line (matops_port.c): Mult,
  *    SolveTrm, RankUpdateSym, the MV* helpers, pivots, unblocked/blocked
  *    Cholesky and LU, and the solves.
