@@ -54,6 +54,27 @@ def test_d_dgemm(flags, m, n, k):
     assert np.all(np.abs(C - ref) <= bound)
 
 
+@pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
+@pytest.mark.parametrize("k", [333, 32, 16])
+def test_d_dgemm_big_tiles(flags, k):
+    """Outputs of >= 512 tiles of 128x128 (the big-tile instance, two k-slabs
+    in flight) with edge tiles in m and n and 21, 2 or 1 slabs of 16 in k."""
+    from matops_amd import _lib
+    m, n = 2900, 2950
+    ta, tb = bool(flags & mo.TRANSA), bool(flags & mo.TRANSB)
+    A = fmat(*((k, m) if ta else (m, k)), seed=44)
+    B = fmat(*((n, k) if tb else (k, n)), seed=45)
+    C0 = fmat(m, n, seed=46)
+    dA, dB, dC = _dev(A), _dev(B), _dev(C0)
+    rc = _lib.lib().mops_d_dgemm(flags, m, n, k, 0.75, dA.data_ptr(), A.shape[0], dB.data_ptr(),
+                                 B.shape[0], -1.0, dC.data_ptr(), m, _stream())
+    assert rc == 0, _lib.lib().mops_last_error()
+    C = _host(dC)
+    ref = 0.75 * (_op(A, ta) @ _op(B, tb)) - C0
+    bound = 0.75 * gemm_bound(np.abs(_op(A, ta)), np.abs(_op(B, tb)), k) + 2 * EPS * np.abs(C0)
+    assert np.all(np.abs(C - ref) <= bound)
+
+
 def test_d_dgemm_bad_arguments():
     from matops_amd import _lib
     L = _lib.lib()
