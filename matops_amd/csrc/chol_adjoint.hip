@@ -71,17 +71,19 @@ __global__ __launch_bounds__(256) void k_chol_adj_diag(int jb, const double* __r
     }
     __syncthreads();
     const double db = dbar;
-    // (i) partial sums of cb^T Bp for row j
+    // one pass over the rows below j: thread (k, g) takes column k < j and
+    // rows j+1+g, j+5+g, ...; (i) its partial sum of cb^T Bp for row j and
+    // (ii) Bpb -= cb r on its elements (column j is only read here)
     double acc = 0.0;
-    if (k < j)
-      for (int i = j + 1 + g; i < jb; i += 4) acc += Bs[i * ADJ_LD + j] * Ls[i * ADJ_LD + k];
-    part[g][k] = acc;
-    // (ii) Bpb -= cb r (rows > j, columns < j)
-    const int nr = jb - j - 1;
-    for (int idx = t; idx < nr * j; idx += 256) {
-      const int i = j + 1 + idx / j, kk = idx % j;
-      Bs[i * ADJ_LD + kk] -= Bs[i * ADJ_LD + j] * Ls[j * ADJ_LD + kk];
+    if (k < j) {
+      const double lj = Ls[j * ADJ_LD + k];
+      for (int i = j + 1 + g; i < jb; i += 4) {
+        const double b = Bs[i * ADJ_LD + j];
+        acc += b * Ls[i * ADJ_LD + k];
+        Bs[i * ADJ_LD + k] -= b * lj;
+      }
     }
+    part[g][k] = acc;
     __syncthreads();
     // row j and the diagonal; wave 0 reads them next, in program order
     if (t < j)
