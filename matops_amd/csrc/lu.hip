@@ -44,6 +44,20 @@ namespace {
 constexpr int LEAF_W = 32;    // leaf width (columns)
 constexpr int LEAF_R = 512;   // rows per workgroup in a leaf (= threads)
 constexpr int LEAF_GMAX = 64;
+constexpr int LU_NBMAX = 512;  // widest panel (columns)
+
+// Panel width of the GPU schedule for blocking nb and min(m, n) = mn.
+// Every blocking gives the same factorization (same pivots, to rounding):
+// a blocking below 256 is used as given; 0 (the reference's unblocked
+// case) and 256 or more take wmax. The FP64 factorization uses LU_NBMAX
+// (n = 16384: 131 ms against 138.5 ms with 256-column panels); the mixed
+// one 256, as its FP32 trailing updates are cheap and wider panels only
+// lengthen the panel chain (FP32 factorization 118 -> 125 ms at 512).
+constexpr int LU_NB_MIXED = 256;
+inline int lu_panel_width(int nb, int mn, int wmax) {
+  const int w = (nb <= 0 || nb >= 256) ? wmax : nb;
+  return w < mn ? w : mn;
+}
 constexpr int LEAF_SW = LEAF_W + 3;  // exchange slot: value, row, row data, perm
 constexpr unsigned SPIN_LIMIT = 1u << 26;
 // Panel state (row movement, inverse of the unit triangle) rotates over
@@ -105,7 +119,7 @@ struct LeafArgs {
   int* perm;       // perm[i] = original row now at position i
   double* xa;      // exchange records: a ring of 3 areas of LEAF_XAREA doubles
   int ring0;       // ring position of the leaf's first column
-  double* stage;   // moved values of the other panel columns: [2 * LEAF_W * 256]
+  double* stage;   // moved values of the other panel columns: [2 * LEAF_W * LU_NBMAX]
   unsigned* ctr;   // grid barrier counter (monotonic over the factorization)
   unsigned base;   // its value when this leaf starts
   int* err;        // set to 1 if a barrier wait timed out
@@ -500,10 +514,11 @@ __global__ void k_perm_pairs(const int* perm, const int* inv0, int k, int m, int
 }
 
 // Moves the rows listed in pairs (row dst of A receives old row src) in the
-// columns [c_lo, c_hi) of A, i.e. T-rows: every workgroup takes 16 of them,
-// reads all source values into LDS, then writes the destinations.
-constexpr int AP_ROWS = 16;
-template <typename D>
+// columns [c_lo, c_hi) of A, i.e. T-rows: every workgroup takes AP_ROWS of
+// them, reads all source values into LDS, then writes the destinations.
+// AP_ROWS = 16, or 8 for panels wider than 256 columns (their up to 1024
+// pairs keep the staging within 64 KiB).
+template <typename D, int AP_ROWS>
 __global__ __launch_bounds__(256) void k_apply_pairs(D* T, long ldT, const int* pairs,
                                                      int maxpairs, int c_lo, int c_hi) {
   extern __shared__ double vals_raw[];  // [maxpairs][AP_ROWS]
@@ -602,8 +617,8 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
     set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
     return ST_BADARG;
   }
-  if (pc1 - pc0 > 256) {
-    set_error("getrf: panel wider than 256 columns");
+  if (pc1 - pc0 > LU_NBMAX) {
+    set_error("getrf: panel wider than %d columns", LU_NBMAX);
     return ST_BADARG;
   }
   LeafArgs a;
@@ -676,9 +691,15 @@ int panel(hipStream_t s, LuCtx& c, int k, int kb, int par) {
 
 int apply_rows(hipStream_t s, LuCtx& c, int par, int c_lo, int c_hi) {
   if (c_hi <= c_lo) return ST_OK;
-  const size_t shm = (size_t)c.maxpairs * AP_ROWS * sizeof(double);
-  hipLaunchKernelGGL(k_apply_pairs<double>, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm,
+  if (c.maxpairs <= 512) {
+    const size_t shm = (size_t)c.maxpairs * 16 * sizeof(double);
+    hipLaunchKernelGGL((k_apply_pairs<double, 16>), dim3(ceil_div(c_hi - c_lo, 16)), dim3(256), shm,
                      s, c.T, c.ldT, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  } else {
+    const size_t shm = (size_t)c.maxpairs * 8 * sizeof(double);
+    hipLaunchKernelGGL((k_apply_pairs<double, 8>), dim3(ceil_div(c_hi - c_lo, 8)), dim3(256), shm,
+                     s, c.T, c.ldT, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  }
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }
@@ -739,12 +760,11 @@ int join_bulk(hipStream_t s, hipStream_t sr) {
 size_t getrf_ws_bytes(int m, int n, int nb) {
   if (m <= 0 || n <= 0) return 0;
   const int mn = m < n ? m : n;
-  if (nb <= 0 || nb > 256) nb = 256;
-  if (nb > mn) nb = mn;
+  nb = lu_panel_width(nb, mn, LU_NBMAX);
   const int G = ceil_div(m, LEAF_R);
   size_t b = (size_t)m * n * sizeof(double);                   // T
   b += (size_t)LEAF_RING * LEAF_XAREA * sizeof(double) + 4096;   // exchange
-  b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
+  b += (size_t)2 * LEAF_W * LU_NBMAX * sizeof(double) + 256;        // gather staging
   b += (size_t)(mn + 3 * m + 64) * sizeof(int);                // piv, perm, inv0, ctr, err
   b += (size_t)LU_NPAR * (4 * nb + 2) * sizeof(int);           // pairs
   b += (size_t)LU_NPAR * nb * nb * sizeof(double);             // linv
@@ -759,9 +779,8 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   const int mn = m < n ? m : n;
   // DecomposeLU: unblocked iff min(m,n) <= nb || nb == 0 (lu.go:267). Every
   // blocking gives the same factorization (same pivots, to rounding), so the
-  // GPU schedule uses panels of at most 256 columns whatever nb is.
-  if (nb <= 0 || nb > 256) nb = 256;
-  if (nb > mn) nb = mn;
+  // GPU schedule picks its own panel width (lu_panel_width).
+  nb = lu_panel_width(nb, mn, LU_NBMAX);
   const int G = ceil_div(m, LEAF_R);
   if (G > LEAF_GMAX) {
     set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
@@ -775,7 +794,7 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
   c.T = (double*)ws_carve((size_t)m * n * sizeof(double));
   c.xa = (double*)ws_carve((size_t)LEAF_RING * LEAF_XAREA * sizeof(double));
   c.col_count = 0;
-  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
+  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * LU_NBMAX * sizeof(double));
   c.piv = (int*)ws_carve((size_t)mn * sizeof(int));
   c.perm = (int*)ws_carve((size_t)m * sizeof(int));
   c.inv0 = (int*)ws_carve((size_t)m * sizeof(int));
@@ -892,9 +911,15 @@ struct MixedCtx {
 
 int apply_rows32(hipStream_t s, LuCtx& c, MixedCtx& x, int par, int c_lo, int c_hi) {
   if (c_hi <= c_lo) return ST_OK;
-  const size_t shm = (size_t)c.maxpairs * AP_ROWS * sizeof(float);
-  hipLaunchKernelGGL(k_apply_pairs<float>, dim3(ceil_div(c_hi - c_lo, AP_ROWS)), dim3(256), shm,
+  if (c.maxpairs <= 512) {
+    const size_t shm = (size_t)c.maxpairs * 16 * sizeof(float);
+    hipLaunchKernelGGL((k_apply_pairs<float, 16>), dim3(ceil_div(c_hi - c_lo, 16)), dim3(256), shm,
                      s, x.T32, (long)c.n, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  } else {
+    const size_t shm = (size_t)c.maxpairs * 8 * sizeof(float);
+    hipLaunchKernelGGL((k_apply_pairs<float, 8>), dim3(ceil_div(c_hi - c_lo, 8)), dim3(256), shm,
+                     s, x.T32, (long)c.n, c.pairs[par], c.maxpairs, c_lo, c_hi);
+  }
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }
@@ -939,13 +964,12 @@ int panel_mixed(hipStream_t s, LuCtx& c, MixedCtx& x, int k, int kb, int par) {
 
 size_t getrf_mixed_ws_bytes(int n, int nb) {
   if (n <= 0) return 0;
-  if (nb <= 0 || nb > 256) nb = 256;
-  if (nb > n) nb = n;
+  nb = lu_panel_width(nb, n, LU_NB_MIXED);
   const int G = ceil_div(n, LEAF_R);
   size_t b = (size_t)n * n * sizeof(float);                    // T32
   b += (size_t)nb * n * sizeof(double);                        // P64
   b += (size_t)LEAF_RING * LEAF_XAREA * sizeof(double) + 4096;   // exchange
-  b += (size_t)2 * LEAF_W * 256 * sizeof(double) + 256;        // gather staging
+  b += (size_t)2 * LEAF_W * LU_NBMAX * sizeof(double) + 256;        // gather staging
   b += (size_t)(4 * n + 64) * sizeof(int);                     // piv, perm, inv0, ctr, err
   b += (size_t)LU_NPAR * (4 * nb + 2) * sizeof(int);           // pairs
   b += (size_t)LU_NPAR * nb * nb * (sizeof(double) + sizeof(float));  // linv
@@ -957,8 +981,7 @@ size_t getrf_mixed_ws_bytes(int n, int nb) {
 int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, double* LU, long ldlu,
                   long long* ipiv, int* perm_out, int* err_flag) {
   if (n <= 0) return ST_OK;
-  if (nb <= 0 || nb > 256) nb = 256;
-  if (nb > n) nb = n;
+  nb = lu_panel_width(nb, n, LU_NB_MIXED);
   const int G = ceil_div(n, LEAF_R);
   if (G > LEAF_GMAX) {
     set_error("getrf: more than %d rows are not supported", LEAF_GMAX * LEAF_R);
@@ -975,7 +998,7 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   x.P64 = (double*)ws_carve((size_t)nb * n * sizeof(double));
   c.xa = (double*)ws_carve((size_t)LEAF_RING * LEAF_XAREA * sizeof(double));
   c.col_count = 0;
-  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * 256 * sizeof(double));
+  c.stage = (double*)ws_carve((size_t)2 * LEAF_W * LU_NBMAX * sizeof(double));
   c.piv = (int*)ws_carve((size_t)n * sizeof(int));
   c.perm = (int*)ws_carve((size_t)n * sizeof(int));
   c.inv0 = (int*)ws_carve((size_t)n * sizeof(int));
