@@ -73,7 +73,11 @@ struct GemmArgs {
 
 constexpr int GROUP_M = 8;
 constexpr int SMALL_KS = 2;  // k shares of the 64x64-tile kernel for full outputs (8 waves)
-constexpr int SG = 8;
+// Super-tile edge (in tiles) of the triangle-only updates. 4 rather than 8
+// or 16: a diagonal super-tile leaves SG(SG-1)/2 of its SG^2 slots empty,
+// and the Cholesky n = 8192 measured 7.11 (4), 7.13 (2), 7.22 (8) and
+// 7.72 ms (16).
+constexpr int SG = 4;
 constexpr int NXCD = 8;
 
 // Hardware hands workgroup ids to the XCDs round robin (id % 8). Renumber so
