@@ -589,9 +589,43 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
       hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(tiles * S)), dim3(256), 0, s, kflags,
                          (int)(tiles * S));
       MOPS_CHECK_HIP(hipGetLastError());
-      g.kchunk = kc;
       g.part = part;
       g.kflags = kflags;
+      if (mode == GM_FULL && N <= 16 && M >= 2048 && K >= 2048) {
+        // skinny products (the residual of the refinement, 16 right-hand
+        // sides): 128x16 tiles with 32-deep slabs, two k shares and two
+        // slabs in flight keep 64 KiB of A in flight per CU (one workgroup
+        // per CU). k is split in S2 chunks so that the tm x S2 workgroups
+        // fill whole rounds of the CUs (cost: rounds x CUs / workgroups,
+        // plus 1% per chunk). The workspace carved above (64x64 tiles, S
+        // chunks) holds any S2 <= 2 S: tm <= tiles / 2, tiles of half the
+        // size.
+        const int tm = ceil_div(M, 128);
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu <= 0)
+          ncu = 256;
+        int S2 = 2;
+        double best = 1e30;
+        for (int c = 2; c <= 2 * S && c <= K / 64; ++c) {
+          const long wg = (long)tm * c;
+          const double cost = (double)((wg + ncu - 1) / ncu) * ncu / (double)wg + 0.01 * c;
+          if (cost < best) {
+            best = cost;
+            S2 = c;
+          }
+        }
+        const int kc2 = ceil_div(ceil_div(K, S2), 32) * 32;
+        S2 = ceil_div(K, kc2);
+        if (S2 >= 2) {
+          g.kchunk = kc2;
+          const int st = launch_gemm<T, 128, 16, 32, 2, 1, 2>(s, ta, tb, g, S2, 1);
+          ws_release(mark);
+          return st;
+        }
+      }
+      g.kchunk = kc;
       const int st = (mode == GM_FULL)
                          ? launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, S, 1)
                          : launch_gemm<T, 64, 64, 16, 2, 2, 1>(s, ta, tb, g, S, 1);
