@@ -75,6 +75,27 @@ def test_d_dgemm_big_tiles(flags, k):
     assert np.all(np.abs(C - ref) <= bound)
 
 
+@pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
+@pytest.mark.parametrize("m,n,k", [(3000, 16, 4000), (2100, 5, 2500), (2048, 1, 2048)])
+def test_d_dgemm_skinny(flags, m, n, k):
+    """Outputs of at most 16 columns with a long k (the refinement's
+    residual shape): the 128x16-tile instance with k split into chunks whose
+    partial tiles are added in chunk order."""
+    from matops_amd import _lib
+    ta, tb = bool(flags & mo.TRANSA), bool(flags & mo.TRANSB)
+    A = fmat(*((k, m) if ta else (m, k)), seed=47)
+    B = fmat(*((n, k) if tb else (k, n)), seed=48)
+    C0 = fmat(m, n, seed=49)
+    dA, dB, dC = _dev(A), _dev(B), _dev(C0)
+    rc = _lib.lib().mops_d_dgemm(flags, m, n, k, -1.0, dA.data_ptr(), A.shape[0], dB.data_ptr(),
+                                 B.shape[0], 1.0, dC.data_ptr(), m, _stream())
+    assert rc == 0, _lib.lib().mops_last_error()
+    C = _host(dC)
+    ref = C0 - _op(A, ta) @ _op(B, tb)
+    bound = gemm_bound(np.abs(_op(A, ta)), np.abs(_op(B, tb)), k) + 2 * EPS * np.abs(C0)
+    assert np.all(np.abs(C - ref) <= bound)
+
+
 def test_d_dgemm_bad_arguments():
     from matops_amd import _lib
     L = _lib.lib()
