@@ -18,6 +18,15 @@
  * mops_last_error(). Positive values are LAPACK-style info codes documented
  * per routine. Every call is re-entrant: state (stream, workspace, error
  * text) is kept per host thread.
+ *
+ * Bounded waits. Kernels that wait for other workgroups of the same launch
+ * (the k split of GEMMs with few output tiles, the chained triangular solve)
+ * give up after a spin limit instead of hanging, and then raise a bit in a
+ * sticky status word of the calling thread. The host-memory entry points
+ * read it before they return and fail with a runtime error (text in
+ * mops_last_error()) when a bit is set; the device-resident twins leave it
+ * for mops_d_status(). The factorizations report their own waits through
+ * their info / status arguments as well.
  */
 #ifndef MOPS_H
 #define MOPS_H
@@ -56,6 +65,12 @@ const char *mops_last_error(void);
 int mops_device_count(void);
 /* Frees the calling thread's device workspace and stream. */
 void mops_thread_release(void);
+/* Synchronizes `stream`, then reads and clears the calling thread's status
+ * word: 0 when no internal wait timed out since the last check, else a
+ * runtime failure (negative) naming the wait in mops_last_error(); the
+ * results of the device-resident calls since the last check are then not
+ * valid. No counterpart in the reference (its CPU kernels never wait). */
+int mops_d_status(void *stream);
 
 /* Launch timing for measurement (no counterpart in the reference). When
  * enabled, the trailing updates of the factorizations are bracketed by HIP
@@ -93,6 +108,14 @@ int mops_d_dgemm(int flags, int M, int N, int P, double alpha, const double *A, 
 int mops_dmmat_solve_blk(mops_mdata_t *B, const mops_mdata_t *A, double alpha, int flags, int N,
                          int S, int E, int NB);
 
+/* Replaces dmmat_solve_unb (calgo/cmops.h:203-204, calgo/dmmat_solve.c:400),
+ * the unblocked form of the same solve: identical arguments and results
+ * except for how the triangle is chosen, which follows dmmat_solve_unb:
+ * LOWER selects the lower triangle, otherwise the upper one (the blocked
+ * form tests UPPER first). Host memory, synchronous. */
+int mops_dmmat_solve_unb(mops_mdata_t *B, const mops_mdata_t *A, double alpha, int flags, int N,
+                         int S, int E);
+
 /* Device-resident TRSM: B is N x nrhs (LEFT) or nrhs x N (RIGHT). */
 int mops_d_dtrsm(int flags, int N, int nrhs, double alpha, const double *A, int lda, double *B,
                  int ldb, void *stream);
@@ -105,7 +128,11 @@ int mops_d_dtrsm(int flags, int N, int nrhs, double alpha, const double *A, int 
  * triangle of the block C[S:E, S:E] with rows S:E of A (columns with
  * TRANSA), inner length P. beta scales the whole triangle (BLAS semantics;
  * the reference scales only its diagonal blocks, a difference that only
- * shows for beta != 1, which no caller on the factorization path uses). */
+ * shows for beta != 1, which no caller on the factorization path uses).
+ * With S > 0 the LOWER case also differs: calgo updates C[i, 0:i] from
+ * rows S..S+i of A (calgo/dmmat_rank.c:116-123, two index bases that agree
+ * only for S = 0, the only value RankUpdateSym passes, mmult.go:436);
+ * here the block is the lower triangle of C[S:E, S:E], as for UPPER. */
 int mops_dmmat_rank_blk(mops_mdata_t *C, const mops_mdata_t *A, double alpha, double beta,
                         int flags, int P, int S, int E, int vlen, int NB);
 
@@ -157,7 +184,9 @@ int mops_d_dpotrf_adjoint(int N, int nb, const double *L, int ldl, double *Abar,
  * LAPACK ipiv - 1. Ties between |candidates| go to the lowest row. A zero
  * pivot is not an error (Inf/NaN propagate, as in the reference). Returns
  * -3 with "pivot array < min(A.Rows(),A.Cols())" when npiv < min(M,N).
- * Host memory, synchronous. */
+ * M is limited to 32768 rows (the pivoting leaf keeps one row per thread
+ * of at most 64 workgroups of 512): taller matrices are rejected with a
+ * message; the reference has no such limit. Host memory, synchronous. */
 int mops_dgetrf(mops_mdata_t *A, int64_t *pivots, int npiv, int M, int N, int nb);
 
 /* Replaces SolveLU (lu.go:320-334): B (N x nrhs) = A^-1 B from the factors
@@ -167,11 +196,12 @@ int mops_dgetrf(mops_mdata_t *A, int64_t *pivots, int npiv, int M, int N, int nb
 int mops_dgetrs(mops_mdata_t *B, const mops_mdata_t *A, const int64_t *pivots, int npiv,
                 int flags, int N, int nrhs);
 
-/* Device-resident twins. ipiv: device int64[min(M,N)] (may be NULL); perm:
- * device int[M] (may be NULL) receiving the row permutation (perm[i] =
- * original row now at i); status: device int (may be NULL), 0 on success,
- * 1 if a panel's grid barrier timed out. mops_d_dgetrs uses perm when given,
- * else rebuilds it from the npiv pivots. */
+/* Device-resident twins (same row limit). ipiv: device int64[min(M,N)]
+ * (may be NULL); perm: device int[M] (may be NULL) receiving the row
+ * permutation (perm[i] = original row now at i); status: device int (may
+ * be NULL), 0 on success, 1 if a panel's grid barrier timed out.
+ * mops_d_dgetrs uses perm when given, else rebuilds it from the npiv
+ * pivots. */
 int mops_d_dgetrf(int M, int N, int nb, double *A, int lda, int64_t *ipiv, int *perm,
                   int *status, void *stream);
 int mops_d_dgetrs(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,

@@ -26,9 +26,33 @@ int d2h(double* h, long ldh, const double* d, long ldd, int rows, int cols, hipS
   return ST_OK;
 }
 
-int finish(hipStream_t s) {
-  MOPS_CHECK_HIP(hipStreamSynchronize(s));
-  return ST_OK;
+// B[:, S:E] (LEFT) or B[S:E, :] (RIGHT) = alpha op(A)^-1 B, op(A) N x N:
+// the host-memory solve behind dmmat_solve_blk and dmmat_solve_unb.
+int solve_host(mops_mdata_t* B, const mops_mdata_t* A, double alpha, int flags, int N, int S,
+               int E) {
+  if (E - S <= 0 || N <= 0) return ST_OK;  // calgo/cmops.go:203, dmmat_solve.c:875
+  if (!B || !B->md) return -1;
+  if (!A || !A->md) return -2;
+  const bool right = flags & F_RIGHT;
+  const int nrhs = E - S;
+  hipStream_t s = host_stream();
+  if (!s) {
+    set_error("could not create a HIP stream");
+    return ST_HIP;
+  }
+  const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
+  WsUse wu(s);
+  MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + trsm_ws_bytes(flags, N, nrhs) + 1024));
+  double* dA = (double*)ws_carve(na * sizeof(double));
+  double* dB = (double*)ws_carve(nb * sizeof(double));
+  // LEFT: columns S:E of B (dmmat_solve.c:425); RIGHT: rows S:E (:405)
+  double* hB = right ? B->md + S : B->md + (long)S * B->step;
+  const int brows = right ? nrhs : N, bcols = right ? N : nrhs;
+  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
+  MOPS_TRY(h2d(dB, brows, hB, B->step, brows, bcols, s));
+  MOPS_TRY(trsm_d(s, flags, N, nrhs, alpha, dA, N, dB, brows));
+  MOPS_TRY(d2h(hB, B->step, dB, brows, brows, bcols, s));
+  return take_status(s);
 }
 
 }  // namespace
@@ -47,6 +71,11 @@ int mops_device_count(void) {
 }
 
 void mops_thread_release(void) { thread_release(); }
+
+int mops_d_status(void* stream) {
+  clear_error();
+  return take_status(as_stream(stream));
+}
 
 void mops_trace_enable(int on) { trace_enable(on); }
 
@@ -115,7 +144,7 @@ int mops_dmult_mm_blocked3(mops_mdata_t* C, const mops_mdata_t* A, const mops_md
   MOPS_TRY(gemm_d(s, ta, tb, rows, cols, P, alpha, dA, ta ? P : rows, dB, tb ? cols : P, beta,
                   dC, rows));
   MOPS_TRY(d2h(hC, C->step, dC, rows, rows, cols, s));
-  return finish(s);
+  return take_status(s);
 }
 
 int mops_d_dtrsm(int flags, int N, int nrhs, double alpha, const double* A, int lda, double* B,
@@ -151,29 +180,16 @@ int mops_dmmat_solve_blk(mops_mdata_t* B, const mops_mdata_t* A, double alpha, i
                          int S, int E, int NB) {
   (void)NB;
   clear_error();
-  if (E - S <= 0 || N <= 0) return ST_OK;  // calgo/cmops.go:203, dmmat_solve.c:875
-  if (!B || !B->md) return -1;
-  if (!A || !A->md) return -2;
-  const bool right = flags & F_RIGHT;
-  const int nrhs = E - S;
-  hipStream_t s = host_stream();
-  if (!s) {
-    set_error("could not create a HIP stream");
-    return ST_HIP;
-  }
-  const size_t na = (size_t)N * N, nb = (size_t)N * nrhs;
-  WsUse wu(s);
-  MOPS_TRY(ws_reserve((na + nb) * sizeof(double) + trsm_ws_bytes(flags, N, nrhs) + 1024));
-  double* dA = (double*)ws_carve(na * sizeof(double));
-  double* dB = (double*)ws_carve(nb * sizeof(double));
-  // LEFT: columns S:E of B (dmmat_solve.c:425); RIGHT: rows S:E (:405)
-  double* hB = right ? B->md + S : B->md + (long)S * B->step;
-  const int brows = right ? nrhs : N, bcols = right ? N : nrhs;
-  MOPS_TRY(h2d(dA, N, A->md, A->step, N, N, s));
-  MOPS_TRY(h2d(dB, brows, hB, B->step, brows, bcols, s));
-  MOPS_TRY(trsm_d(s, flags, N, nrhs, alpha, dA, N, dB, brows));
-  MOPS_TRY(d2h(hB, B->step, dB, brows, brows, bcols, s));
-  return finish(s);
+  return solve_host(B, A, alpha, flags, N, S, E);
+}
+
+int mops_dmmat_solve_unb(mops_mdata_t* B, const mops_mdata_t* A, double alpha, int flags, int N,
+                         int S, int E) {
+  clear_error();
+  // dmmat_solve_unb picks the lower triangle iff LOWER is set
+  // (calgo/dmmat_solve.c:409, 423); the blocked kernels test UPPER first
+  const int f = (flags & F_LOWER) ? (flags & ~F_UPPER) : ((flags & ~F_LOWER) | F_UPPER);
+  return solve_host(B, A, alpha, f, N, S, E);
 }
 
 int mops_dmmat_rank_blk(mops_mdata_t* C, const mops_mdata_t* A, double alpha, double beta,
@@ -207,7 +223,7 @@ int mops_dmmat_rank_blk(mops_mdata_t* C, const mops_mdata_t* A, double alpha, do
   const int mode = (flags & F_LOWER) ? GM_LOWER : GM_UPPER;
   MOPS_TRY(gemm_d(s, ta, !ta, n, n, P, alpha, dA, ta ? P : n, dA, ta ? P : n, beta, dC, n, mode));
   MOPS_TRY(d2h(hC, C->step, dC, n, n, n, s));
-  return finish(s);
+  return take_status(s);
 }
 
 }  // extern "C"

@@ -83,6 +83,7 @@ int mops_dpotrf(mops_mdata_t* A, int flags, int N, int nb) {
   MOPS_CHECK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipMemcpyAsync(&bad, dbad, sizeof(double), hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  MOPS_TRY(take_status(s));
   if (info == -2) {
     set_error("Cholesky leaf: wait for the diagonal block timed out");
     return ST_HIP;
@@ -128,8 +129,7 @@ int mops_dpotrf_adjoint(const mops_mdata_t* L, mops_mdata_t* Lbar, int N, int nb
   MOPS_TRY(h2d(dB, N, Lbar->md, Lbar->step, N, N, s));
   MOPS_TRY(chol_adjoint_d(s, N, nb, dL, N, dB, N));
   MOPS_TRY(d2h(Lbar->md, Lbar->step, dB, N, N, N, s));
-  MOPS_CHECK_HIP(hipStreamSynchronize(s));
-  return ST_OK;
+  return take_status(s);
 }
 
 int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nrhs) {
@@ -150,8 +150,7 @@ int mops_dpotrs(mops_mdata_t* B, const mops_mdata_t* A, int flags, int N, int nr
   MOPS_TRY(h2d(dB, N, B->md, B->step, N, nrhs, s));
   MOPS_TRY(potrs_d(s, flags, N, nrhs, dA, N, dB, N));
   MOPS_TRY(d2h(B->md, B->step, dB, N, N, nrhs, s));
-  MOPS_CHECK_HIP(hipStreamSynchronize(s));
-  return ST_OK;
+  return take_status(s);
 }
 
 }  // extern "C"
@@ -222,6 +221,7 @@ int mops_dgetrf(mops_mdata_t* A, int64_t* pivots, int npiv, int M, int N, int nb
                                 hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, s));
   MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  MOPS_TRY(take_status(s));
   if (st != 0) {
     set_error("LU panel: grid barrier timed out");
     return ST_HIP;
@@ -254,8 +254,7 @@ int mops_dgetrs(mops_mdata_t* B, const mops_mdata_t* A, const int64_t* pivots, i
                                   hipMemcpyHostToDevice, s));
   MOPS_TRY(getrs_d(s, flags, N, nrhs, dA, N, dp, np, nullptr, dB, N));
   MOPS_TRY(d2h(B->md, B->step, dB, N, N, nrhs, s));
-  MOPS_CHECK_HIP(hipStreamSynchronize(s));
-  return ST_OK;
+  return take_status(s);
 }
 
 }  // extern "C"
@@ -313,8 +312,7 @@ int mops_dgesv_ir(const mops_mdata_t* A, const mops_mdata_t* B, mops_mdata_t* X,
   MOPS_TRY(h2d(dB, N, B->md, B->step, N, nrhs, s));
   MOPS_TRY(gesv_ir_d(s, N, nrhs, dA, N, dB, N, dX, N, prec, tol, maxit, rep));
   MOPS_TRY(d2h(X->md, X->step, dX, N, N, nrhs, s));
-  MOPS_CHECK_HIP(hipStreamSynchronize(s));
-  return ST_OK;
+  return take_status(s);
 }
 
 }  // extern "C"

@@ -69,6 +69,8 @@ struct GemmArgs {
   int kchunk;
   T* part;
   int* kflags;
+  int* status;  // raised (STATUS_GEMM_WAIT) when a wait for a chunk times out
+  unsigned spin_limit;  // polls of one chunk flag before giving up
 };
 
 constexpr int GROUP_M = 8;
@@ -376,11 +378,17 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     }
     if (tid == 0) {
       for (int zz = 0; zz < S - 1; ++zz) {
-        unsigned spins = 0;
-        while (__hip_atomic_load(&g.kflags[tid_tile * S + zz], __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-               ++spins < (1u << 26))
+        for (unsigned spins = 0;; ++spins) {
+          if (spins >= g.spin_limit) {
+            // give up rather than hang; the host reads the status word
+            raise_status(g.status, STATUS_GEMM_WAIT);
+            break;
+          }
+          if (__hip_atomic_load(&g.kflags[tid_tile * S + zz], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT) != 0)
+            break;
           __builtin_amdgcn_s_sleep(1);
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
@@ -473,6 +481,29 @@ inline int grid_for(long total) {
   return (int)b;
 }
 
+// Scratch of a split-k launch: partial tiles and chunk flags. Carved from
+// the calling thread's workspace, or taken from the buffer of a
+// SplitScratchOverride in scope. False when it does not fit.
+struct SplitScratch {
+  void* part;
+  int* flags;
+};
+thread_local void* g_split_base = nullptr;
+thread_local size_t g_split_bytes = 0;
+
+bool split_scratch(long part_bytes, long nflags, SplitScratch& sc) {
+  const size_t pb = align_up((size_t)part_bytes, 256), fb = align_up((size_t)nflags * 4, 256);
+  if (g_split_base) {
+    if (pb + fb > g_split_bytes) return false;
+    sc.part = g_split_base;
+    sc.flags = (int*)((char*)g_split_base + pb);
+    return true;
+  }
+  sc.part = ws_carve(pb);
+  sc.flags = (int*)ws_carve(fb);
+  return sc.part && sc.flags;
+}
+
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, int KS>
 int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int batch2) {
   g.tiles_m = ceil_div(g.M, BM);
@@ -528,7 +559,10 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
     set_error("gemm: triangular update needs a square C");
     return ST_BADARG;
   }
-  if (K == 0 || alpha == T(0)) {
+  // Only an empty inner dimension skips the product: with alpha == 0 the
+  // product is still formed, so Inf/NaN in op(A) or op(B) reach C as they
+  // do through calgo's dmult_mm_blocked3 (calgo/dmmat_gemm.c:297,413).
+  if (K == 0) {
     if (beta == T(1)) return ST_OK;
     if (batch != 1 || batch2 != 1) {
       set_error("gemm: batched scale-only call is not supported");
@@ -575,60 +609,54 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   g.kchunk = 0;
   g.part = nullptr;
   g.kflags = nullptr;
+  g.status = nullptr;
+  g.spin_limit = 0;
   // Few output tiles and a long k: split k so that the grid still covers
   // the GPU (the partial tiles are added in a fixed order, so the result
   // does not depend on timing). Needs workspace; without it, no split.
   int S = 0, kc = 0;
   if ((mode == GM_FULL || mode == GM_LOWER || mode == GM_UPPER) && tri == TRI_NONE &&
       batch == 1 && batch2 == 1 && gemm_split_k(M, N, K, S, kc)) {
+    // skinny products (the residual of the refinement, 16 right-hand
+    // sides): 128x16 tiles with 32-deep slabs, two k shares and two slabs in
+    // flight keep 64 KiB of A in flight per CU (one workgroup per CU); k is
+    // split in S2 chunks so that the tm x S2 workgroups fill whole rounds of
+    // the CUs. The plan is made before the flags are cleared, so that
+    // exactly the flags this launch polls start at zero.
+    int S2 = 0, kc2 = 0;
+    if (mode == GM_FULL && gemm_skinny(M, N, K)) {
+      int dev = 0, ncu = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          ncu <= 0)
+        ncu = 256;
+      S2 = gemm_skinny_chunks(M, K, S, ncu, kc2);
+    }
+    long part_elems = 0, nflags = 0;
+    gemm_split_need(M, N, K, part_elems, nflags);
+    const long used_flags = S2 >= 2 ? (long)ceil_div(M, 128) * S2
+                                    : (long)ceil_div(M, 64) * ceil_div(N, 64) * S;
     const size_t mark = ws_mark();
-    const long tiles = (long)ceil_div(M, 64) * ceil_div(N, 64);
-    T* part = (T*)ws_carve((size_t)(S - 1) * tiles * 64 * 64 * sizeof(T));
-    int* kflags = (int*)ws_carve((size_t)tiles * S * sizeof(int));
-    if (part && kflags) {
-      hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(tiles * S)), dim3(256), 0, s, kflags,
-                         (int)(tiles * S));
+    SplitScratch sc;
+    int* status = active_status();
+    if (split_scratch(part_elems * (long)sizeof(T), nflags, sc) && status &&
+        used_flags <= nflags) {
+      hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(used_flags)), dim3(256), 0, s, sc.flags,
+                         (int)used_flags);
       MOPS_CHECK_HIP(hipGetLastError());
-      g.part = part;
-      g.kflags = kflags;
-      if (mode == GM_FULL && N <= 16 && M >= 2048 && K >= 2048) {
-        // skinny products (the residual of the refinement, 16 right-hand
-        // sides): 128x16 tiles with 32-deep slabs, two k shares and two
-        // slabs in flight keep 64 KiB of A in flight per CU (one workgroup
-        // per CU). k is split in S2 chunks so that the tm x S2 workgroups
-        // fill whole rounds of the CUs (cost: rounds x CUs / workgroups,
-        // plus 1% per chunk). The workspace carved above (64x64 tiles, S
-        // chunks) holds any S2 <= 2 S: tm <= tiles / 2, tiles of half the
-        // size.
-        const int tm = ceil_div(M, 128);
-        int dev = 0, ncu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            ncu <= 0)
-          ncu = 256;
-        int S2 = 2;
-        double best = 1e30;
-        for (int c = 2; c <= 2 * S && c <= K / 64; ++c) {
-          const long wg = (long)tm * c;
-          const double cost = (double)((wg + ncu - 1) / ncu) * ncu / (double)wg + 0.01 * c;
-          if (cost < best) {
-            best = cost;
-            S2 = c;
-          }
-        }
-        const int kc2 = ceil_div(ceil_div(K, S2), 32) * 32;
-        S2 = ceil_div(K, kc2);
-        if (S2 >= 2) {
-          g.kchunk = kc2;
-          const int st = launch_gemm<T, 128, 16, 32, 2, 1, 2>(s, ta, tb, g, S2, 1);
-          ws_release(mark);
-          return st;
-        }
+      g.part = (T*)sc.part;
+      g.kflags = sc.flags;
+      g.status = status;
+      g.spin_limit = wait_spin_limit(1u << 26);
+      int st;
+      if (S2 >= 2) {
+        g.kchunk = kc2;
+        st = launch_gemm<T, 128, 16, 32, 2, 1, 2>(s, ta, tb, g, S2, 1);
+      } else {
+        g.kchunk = kc;
+        st = (mode == GM_FULL) ? launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, S, 1)
+                               : launch_gemm<T, 64, 64, 16, 2, 2, 1>(s, ta, tb, g, S, 1);
       }
-      g.kchunk = kc;
-      const int st = (mode == GM_FULL)
-                         ? launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, S, 1)
-                         : launch_gemm<T, 64, 64, 16, 2, 2, 1>(s, ta, tb, g, S, 1);
       ws_release(mark);
       return st;
     }
@@ -648,6 +676,17 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
 }
 
 }  // namespace
+
+SplitScratchOverride::SplitScratchOverride(void* base, size_t bytes)
+    : prev_base_(g_split_base), prev_bytes_(g_split_bytes) {
+  g_split_base = base;
+  g_split_bytes = bytes;
+}
+
+SplitScratchOverride::~SplitScratchOverride() {
+  g_split_base = prev_base_;
+  g_split_bytes = prev_bytes_;
+}
 
 int gemm_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha, const double* A,
            long lda, const double* B, long ldb, double beta, double* C, long ldc, int mode,

@@ -44,20 +44,25 @@ __global__ void k_inf_norm_finish(int n, int nch, const double* __restrict__ par
   if (i >= n) return;
   double s = 0.0;
   for (int c = 0; c < nch; ++c) s += part[(long)c * n + i];
-  atomic_max_nonneg(out, s);
+  // a NaN row sum has larger bits than +Inf, so the maximum keeps it
+  atomic_max_nonneg(out, s != s ? __longlong_as_double(0x7ff8000000000000ll) : s);
 }
 
-// out[j] = max_i |M[i][j]|; workgroup = column
+// NaN-propagating maximum (fmax drops a NaN operand): a NaN anywhere makes
+// the result NaN, so a non-finite residual or solution is never hidden.
+__device__ __forceinline__ double max_nan(double m, double v) { return (v > m || v != v) ? v : m; }
+
+// out[j] = max_i |M[i][j]| (NaN if any entry is NaN); workgroup = column
 __global__ __launch_bounds__(256) void k_col_absmax(int n, const double* __restrict__ M, long ld,
                                                     double* out) {
   __shared__ double red[256];
   const int j = blockIdx.x;
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(M[i + (long)j * ld]));
+  for (int i = threadIdx.x; i < n; i += 256) m = max_nan(m, fabs(M[i + (long)j * ld]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    if (threadIdx.x < w) red[threadIdx.x] = max_nan(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
   if (threadIdx.x == 0) out[j] = red[0];
@@ -82,16 +87,22 @@ __global__ void k_zero_tile(int n, int nrhs, double* X, long ldx) {
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-// eta = max_j rmax_j / (anorm xmax_j + bmax_j)
+// eta = max_j rmax_j / (anorm xmax_j + bmax_j); +Inf when any of the norms
+// is not finite (a singular A gives Inf/NaN in X, which must never read as
+// converged)
 __global__ void k_eta(int nrhs, const double* rmax, const double* xmax, const double* bmax,
                       const unsigned long long* anorm_bits, double* eta) {
   if (threadIdx.x != 0) return;
   const double an = __longlong_as_double((long long)*anorm_bits);
-  double e = 0.0;
+  double e = isfinite(an) ? 0.0 : INFINITY;
   for (int j = 0; j < nrhs; ++j) {
+    if (!isfinite(rmax[j]) || !isfinite(xmax[j]) || !isfinite(bmax[j])) {
+      e = INFINITY;
+      continue;
+    }
     const double den = an * xmax[j] + bmax[j];
     const double q = den > 0.0 ? rmax[j] / den : (rmax[j] > 0.0 ? INFINITY : 0.0);
-    e = fmax(e, q);
+    e = max_nan(e, q);
   }
   *eta = e;
 }

@@ -97,6 +97,40 @@ class WsUse {
   hipStream_t s_;
 };
 
+// Sticky status word of the calling thread on the current device (device
+// int, zero until a kernel raises a bit; nullptr if it cannot be allocated).
+// Kernels whose bounded waits have no other way to report a timeout OR
+// their bit into it; take_status() reads and clears it.
+enum : int { STATUS_GEMM_WAIT = 1, STATUS_TRSM_WAIT = 2 };
+int* status_word();
+// Synchronizes s, then reads and clears the status word: ST_OK when no bit
+// was raised, else ST_HIP with the text naming the wait that timed out.
+int take_status(hipStream_t s);
+// The status word kernels of the calling thread write to: status_word(),
+// or the pointer installed by a StatusOverride in scope (entry points that
+// may not allocate pass one inside their own scratch).
+int* active_status();
+class StatusOverride {
+ public:
+  explicit StatusOverride(int* p);
+  ~StatusOverride();
+  StatusOverride(const StatusOverride&) = delete;
+  StatusOverride& operator=(const StatusOverride&) = delete;
+
+ private:
+  int* prev_;
+};
+
+// Spin limit of the bounded waits: `dflt`, unless the environment variable
+// MOPS_SPIN_LIMIT (read once per process) gives another value. A limit of 0
+// makes every bounded wait give up at once, which lets tests of the failure
+// path run without any kernel waiting at all.
+unsigned wait_spin_limit(unsigned dflt);
+
+__device__ __forceinline__ void raise_status(int* p, int bit) {
+  if (p) __hip_atomic_fetch_or(p, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Per-thread stream used by the host-pointer entry points.
 hipStream_t host_stream();
 // Per-thread second stream (default priority) for look-ahead work, and six

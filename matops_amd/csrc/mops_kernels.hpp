@@ -2,6 +2,7 @@
 // Every routine is asynchronous on `s` and returns a status code.
 #pragma once
 
+#include "gemm_plan.hpp"
 #include "mops_common.hpp"
 
 namespace mops {
@@ -19,40 +20,21 @@ enum : int { GM_FULL = 0, GM_LOWER = 1, GM_UPPER = 2, GM_LOWER_T = 3, GM_UPPER_T
 //   TRI_B_LOWER: op(B)[k][j] == 0 for k < j      TRI_B_UPPER: k > j
 enum : int { TRI_NONE = 0, TRI_A_LOWER = 1, TRI_A_UPPER = 2, TRI_B_LOWER = 3, TRI_B_UPPER = 4 };
 
-// Tile configuration rule of gemm_d / gemm_s: 128x128 tiles when the output
-// has at least 512 of them, else 64x64 tiles (so the grid still covers the
-// 256 CUs).
-inline bool gemm_big_tiles(long M, long N, long batch = 1) {
-  return ((M + 127) / 128) * ((N + 127) / 128) * batch >= 512;
-}
+// While one is in scope, the split-k launches of gemm_d / gemm_s on the
+// calling thread take their partial tiles and flags from this device buffer
+// (bytes long, 256-byte aligned) instead of the workspace; a split that
+// does not fit then runs unsplit.
+class SplitScratchOverride {
+ public:
+  SplitScratchOverride(void* base, size_t bytes);
+  ~SplitScratchOverride();
+  SplitScratchOverride(const SplitScratchOverride&) = delete;
+  SplitScratchOverride& operator=(const SplitScratchOverride&) = delete;
 
-// Split of k for outputs of few 64x64 tiles and a long k (fewer than 128
-// tiles and k >= 512, or fewer than 512 tiles and k >= 2048): S chunks of
-// kc (a multiple of 16, at least 128) so that S x tiles comes near 1024
-// workgroups. Returns false when no split applies.
-inline bool gemm_split_k(long M, long N, long K, int& S, int& kc) {
-  const long tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  if (tiles <= 0 || tiles >= 512 || K < 512) return false;
-  if (tiles >= 128 && K < 2048) return false;
-  long s = (1024 + tiles - 1) / tiles;
-  if (s > K / 128) s = K / 128;
-  if (s > 32) s = 32;
-  if (s < 2) return false;
-  long c = (K + s - 1) / s;
-  c = (c + 15) / 16 * 16;
-  S = (int)((K + c - 1) / c);
-  kc = (int)c;
-  return S >= 2;
-}
-
-// Workspace the split of gemm_d / gemm_s may carve (bytes, for elements of
-// elem_bytes); 0 when the shape is not split.
-inline size_t gemm_split_ws_bytes(long M, long N, long K, size_t elem_bytes) {
-  int S = 0, kc = 0;
-  if (!gemm_split_k(M, N, K, S, kc)) return 0;
-  const long tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  return (size_t)(S - 1) * tiles * 64 * 64 * elem_bytes + (size_t)tiles * S * sizeof(int) + 1024;
-}
+ private:
+  void* prev_base_;
+  size_t prev_bytes_;
+};
 
 // C(MxN) = alpha*op(A)*op(B) + beta*C, column-major. beta == 0 never reads C
 // (NaN/Inf in C are cleared, as dscale_tile does at calgo/vec_scale.c:114).

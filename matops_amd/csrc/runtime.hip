@@ -4,8 +4,10 @@
 // The reference calls its kernels concurrently from several goroutines on
 // disjoint blocks of C (mmult.go:241-255), so nothing here is shared between
 // host threads: each thread owns its stream and workspace.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <hip/hip_ext.h>
@@ -24,11 +26,17 @@ struct ThreadState {
   hipStream_t bulk[16] = {nullptr};
   bool bulk_tried[16] = {};
   hipEvent_t ev[16][6] = {};
+  int* status[16] = {};                 // sticky status words (device ints)
   hipEvent_t ws_done[16] = {};          // end of the last call's work
   hipStream_t ws_done_stream[16] = {};  // the stream it was recorded on
   bool ws_done_set[16] = {};
 };
 thread_local ThreadState g_ts;
+thread_local int* g_status_override = nullptr;
+
+__global__ void k_clear_status(int* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
 
 int current_device() {
   int d = 0;
@@ -124,16 +132,79 @@ void ws_release(size_t mark) {
   if (mark <= w.used) w.used = mark;
 }
 
+int* status_word() {
+  const int d = current_device();
+  if (!g_ts.status[d]) {
+    void* p = nullptr;
+    if (hipMalloc(&p, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    // synchronous on purpose: the word is zero before any kernel can see it
+    if (hipMemset(p, 0, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(p);
+      return nullptr;
+    }
+    g_ts.status[d] = static_cast<int*>(p);
+  }
+  return g_ts.status[d];
+}
+
+unsigned wait_spin_limit(unsigned dflt) {
+  // -1: not read yet, -2: not set
+  static std::atomic<long long> g_limit{-1};
+  long long v = g_limit.load(std::memory_order_relaxed);
+  if (v == -1) {
+    const char* e = getenv("MOPS_SPIN_LIMIT");
+    v = -2;
+    if (e && *e) {
+      char* end = nullptr;
+      const unsigned long long x = strtoull(e, &end, 10);
+      if (end && *end == 0 && x <= 0xffffffffull) v = (long long)x;
+    }
+    g_limit.store(v, std::memory_order_relaxed);
+  }
+  return v >= 0 ? (unsigned)v : dflt;
+}
+
+int* active_status() { return g_status_override ? g_status_override : status_word(); }
+
+StatusOverride::StatusOverride(int* p) : prev_(g_status_override) { g_status_override = p; }
+StatusOverride::~StatusOverride() { g_status_override = prev_; }
+
+int take_status(hipStream_t s) {
+  int* p = g_status_override ? g_status_override : g_ts.status[current_device()];
+  if (!p) return ST_OK;  // never allocated: no kernel could have raised a bit
+  int v = 0;
+  MOPS_CHECK_HIP(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, s));
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  if (v == 0) return ST_OK;
+  hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, s, p);
+  MOPS_CHECK_HIP(hipGetLastError());
+  MOPS_CHECK_HIP(hipStreamSynchronize(s));
+  set_error("an internal wait timed out (%s%s); the results of the calls since the last "
+            "status check are not valid",
+            (v & STATUS_GEMM_WAIT) ? "split-k GEMM partial tiles" : "",
+            (v & STATUS_TRSM_WAIT) ? ((v & STATUS_GEMM_WAIT) ? ", chained triangular solve"
+                                                             : "chained triangular solve")
+                                   : "");
+  return ST_HIP;
+}
+
 void thread_release() {
   int cur = 0;
   (void)hipGetDevice(&cur);
   for (int d = 0; d < 16; ++d) {
-    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d]) continue;
+    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d] && !g_ts.status[d])
+      continue;
     if (hipSetDevice(d) != hipSuccess) continue;
     if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
     // the last call's work (on whatever stream) is done with the workspace
     if (g_ts.ws_done_set[d]) (void)hipEventSynchronize(g_ts.ws_done[d]);
     if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
+    if (g_ts.status[d]) (void)hipFree(g_ts.status[d]);
+    g_ts.status[d] = nullptr;
     if (g_ts.stream[d]) (void)hipStreamDestroy(g_ts.stream[d]);
     if (g_ts.aux[d]) {
       (void)hipStreamSynchronize(g_ts.aux[d]);

@@ -108,13 +108,15 @@ __device__ __forceinline__ bool tc_is_sent(double v) {
   return (unsigned long long)__double_as_longlong(v) == TC_SENT;
 }
 
-__device__ __forceinline__ void tc_wait_flag(const int* flag, int* err) {
-  unsigned spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    if (++spins > TC_SPIN_LIMIT) {
-      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Waits until *flag != 0 for at most `limit` polls; then raises the status
+// bit and gives up (limit 0: gives up at once, without polling).
+__device__ __forceinline__ void tc_wait_flag(const int* flag, int* err, unsigned limit) {
+  for (unsigned spins = 0;; ++spins) {
+    if (spins >= limit) {
+      raise_status(err, STATUS_TRSM_WAIT);
       break;
     }
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
     __builtin_amdgcn_s_sleep(1);
   }
 }
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
                                                      const double* __restrict__ A, long lda,
                                                      const double* __restrict__ Dinv, double* B,
                                                      long ldb, double* W, long ldw, int* flags,
-                                                     int* err) {
+                                                     int* err, unsigned spin_limit) {
   __shared__ double As[TC_BS * TC_LDA];  // op(A) block [row][k]
   __shared__ double Xs[TC_BS * TC_LDX];  // X block [k][col]
   __shared__ double Ys[TC_BS * TC_LDX];  // acc [k][col]
@@ -193,16 +195,15 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
                      : 0.0;
     }
     if (poll) {
-      unsigned spins = 0;
-      for (;;) {
+      for (unsigned spins = 0;; ++spins) {
+        if (spins >= spin_limit) {
+          raise_status(err, STATUS_TRSM_WAIT);
+          break;
+        }
         bool pending = false;
 #pragma unroll
         for (int q = 0; q < XPER; ++q) pending |= need[q] && tc_is_sent(v[q]);
         if (!pending) break;
-        if (++spins > TC_SPIN_LIMIT) {
-          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
         __builtin_amdgcn_s_sleep(1);
 #pragma unroll
         for (int q = 0; q < XPER; ++q) {
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
         direct_fetch(blk(it + 1), pre);
     }
     if (!last) {
-      if (t == 0) tc_wait_flag(&flags[c * ngrp + g], err);
+      if (t == 0) tc_wait_flag(&flags[c * ngrp + g], err, spin_limit);
       __syncthreads();
     }
     load_x(c, last);
@@ -514,13 +515,18 @@ int chain_solve(hipStream_t s, int flags, int N, int nrhs, const double* A, long
   hipLaunchKernelGGL(k_fill_u64, dim3((int)std::min<long>((nw + 255) / 256, 4096)), dim3(256), 0,
                      s, (unsigned long long*)w.W, nw, TC_SENT);
   const dim3 grid(w.nb64 * w.ngrp), block(TC_T);
-  int* err = w.fl + w.nb64 * w.ngrp;
+  // a wait that gives up raises STATUS_TRSM_WAIT in the caller's status word
+  int* err = active_status();
+  if (!err) {
+    set_error("trsm: status word could not be allocated");
+    return ST_NOMEM;
+  }
   if (trans)
     hipLaunchKernelGGL(k_trsm_chain<true>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
-                       w.D, B, ldb, w.W, w.ldw, w.fl, err);
+                       w.D, B, ldb, w.W, w.ldw, w.fl, err, wait_spin_limit(TC_SPIN_LIMIT));
   else
     hipLaunchKernelGGL(k_trsm_chain<false>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
-                       w.D, B, ldb, w.W, w.ldw, w.fl, err);
+                       w.D, B, ldb, w.W, w.ldw, w.fl, err, wait_spin_limit(TC_SPIN_LIMIT));
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }

@@ -13,6 +13,8 @@ Everything here computes on the GPU through libmops.so; there is no CPU
 fallback.
 """
 import ctypes
+import math
+import threading
 
 import numpy as np
 
@@ -34,6 +36,7 @@ NONE = 0
 # ignores them; they are kept so BlockingParams keeps its meaning.
 _vpLen, _nB, _mB = 196, 68, 68
 _nWorker = 1
+_limitOne = 200 * 200 * 200  # problems below this run in one call (mmult.go:39)
 _panicOnError = False
 
 
@@ -55,12 +58,61 @@ def BlockingParams(mb, nb, kb):
 
 
 def NumWorkers(newWorkers):
-    """mmult.go:69-73. Returns the previous value. The GPU path does not
-    split work over host threads, so the value is only remembered."""
+    """mmult.go:69-73. Returns the previous value. With more than one
+    worker, Mult splits C into blocks and calls the kernel-level entry point
+    from that many host threads at once, as the Go code starts goroutines
+    (mmult.go:246-254); each thread has its own HIP stream and workspace."""
     global _nWorker
     old = _nWorker
     _nWorker = newWorkers
     return old
+
+
+def blockIndex4(i, r, sz):
+    """mmult.go:105-110: start of block i of r over sz, rounded down to a
+    multiple of 4 (the last block ends at sz)."""
+    if i == r:
+        return sz
+    v = i * sz // r
+    return v - (v & 0x3)
+
+
+def divideWork(rows, cols, workers):
+    """mmult.go:137-167: (column workers, row workers) for a C of rows x
+    cols."""
+    nwsqrt = int(math.sqrt(float(workers)))
+    issquare = nwsqrt * nwsqrt == workers
+    if workers == 2 or (workers & 0x1) != 0:
+        return (workers, 1) if cols > rows else (1, workers)
+    if issquare:
+        return nwsqrt, nwsqrt
+    return (workers // 2, 2) if cols > rows else (2, workers // 2)
+
+
+def scheduleWork(colworks, rowworks, cols, rows, worker):
+    """mmult.go:171-189: runs worker(colstart, colend, rowstart, rowend) for
+    every block at once, each on its own thread, and waits for all. The
+    first failure is re-raised."""
+    errs = []
+
+    def run(*blk):
+        try:
+            worker(*blk)
+        except Exception as e:  # noqa: BLE001 - re-raised below
+            errs.append(e)
+
+    ths = []
+    for k in range(colworks):
+        cs, ce = blockIndex4(k, colworks, cols), blockIndex4(k + 1, colworks, cols)
+        for l in range(rowworks):
+            rs, re_ = blockIndex4(l, rowworks, rows), blockIndex4(l + 1, rowworks, rows)
+            t = threading.Thread(target=run, args=(cs, ce, rs, re_))
+            t.start()
+            ths.append(t)
+    for t in ths:
+        t.join()
+    if errs:
+        raise errs[0]
 
 
 def _onError(msg):
@@ -106,12 +158,22 @@ def Mult(C, A, B, alpha, beta, flags):
     if not ok:
         return MatopsError("Mult: size mismatch")
     P = A.shape[0] if flags & TRANSA else A.shape[1]
+    psize = C.size * A.shape[1]  # as mmult.go:227 counts it
     Cm, _, _ = _view(C)
     Am, _, _ = _view(A)
     Bm, _, _ = _view(B)
-    check("mops_dmult_mm_blocked3", _lib.lib().mops_dmult_mm_blocked3(
-        ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags,
-        P, 0, cc, 0, cr, _vpLen, _nB, _mB))
+    L = _lib.lib()
+
+    def worker(cs, ce, rs, re_):
+        check("mops_dmult_mm_blocked3", L.mops_dmult_mm_blocked3(
+            ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags,
+            P, cs, ce, rs, re_, _vpLen, _nB, _mB))
+
+    if _nWorker <= 1 or psize <= _limitOne:
+        worker(0, cc, 0, cr)
+        return None
+    colworks, rowworks = divideWork(cr, cc, _nWorker)
+    scheduleWork(colworks, rowworks, cc, cr, worker)
     return None
 
 

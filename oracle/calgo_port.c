@@ -299,8 +299,24 @@ static void p_dvec_invscal(o_mvec_t *X, double alpha, int N) {
   for (int i = 0; i < N; i++) X->md[(ptrdiff_t)i * X->inc] /= alpha;
 }
 
+/* dmmat_solve_unb, calgo/dmmat_solve.c:400-440: alpha is applied first
+ * (dscale_tile of the block, :405-408 and :419-422), then the scalar
+ * substitution; the lower triangle is used iff LOWER is set (:409, :423),
+ * where the blocked form tests UPPER first. */
+static void p_dmmat_solve_unb(o_mdata_t *B, const o_mdata_t *A, double alpha, int flags, int N,
+                              int S, int E) {
+  if (E - S <= 0) return;
+  if (flags & O_RIGHT) {
+    if (alpha != 1.0) p_scale_tile(B->md + S, B->step, alpha, E - S, N);
+  } else {
+    if (alpha != 1.0) p_scale_tile(B->md + (ptrdiff_t)S * B->step, B->step, alpha, N, E - S);
+  }
+  const int f = (flags & O_LOWER) ? (flags & ~O_UPPER) : ((flags & ~O_LOWER) | O_UPPER);
+  p_dmmat_solve_blk(B, A, 1.0, f, N, S, E, 0);
+}
+
 const okern_t o_port_kernels = {
     p_dmult_mm_blocked3, p_dmmat_solve_blk,    p_dmmat_rank_blk, p_dmvec_symv_rank,
     p_dmvec_solve_blocked, p_dmult_gemv_blocked, p_dvec_dot,       p_dvec_swap,
-    p_dvec_invscal,
+    p_dvec_invscal,      p_dmmat_solve_unb,
 };

@@ -53,6 +53,7 @@ int o_use_kernels(int which, const char *path) {
   SYM(dvec_dot)
   SYM(dvec_swap)
   SYM(dvec_invscal)
+  SYM(dmmat_solve_unb)
 #undef SYM
   K = k;
   K_init = 1;
@@ -65,7 +66,8 @@ int o_kernels_in_use(void) { return K_which; }
 /* Mixes the two kernel sets: bit i of `mask` takes the i-th okern_t entry
  * from the reference (0 dmult_mm_blocked3, 1 dmmat_solve_blk, 2
  * dmmat_rank_blk, 3 dmvec_symv_rank, 4 dmvec_solve_blocked, 5
- * dmult_gemv_blocked, 6 dvec_dot, 7 dvec_swap, 8 dvec_invscal). Used to
+ * dmult_gemv_blocked, 6 dvec_dot, 7 dvec_swap, 8 dvec_invscal, 9
+ * dmmat_solve_unb). Used to
  * localise differences between the restatement and the reference. */
 int o_use_kernel_mask(int mask, const char *path) {
   if (o_use_kernels(1, path) != 0) return -1;
@@ -504,4 +506,29 @@ void o_chol_adjoint(int n, const double *L, int ldl, const double *Lbar, int ldb
   free(Lt);
   free(Lb);
   free(P);
+}
+
+/* ---- the kernel-level boundary (calgo/cmops.go:68-91, 171-240) ---- */
+void o_kern_gemm(double *C, int ldc, const double *A, int lda, const double *B, int ldb,
+                 double alpha, double beta, int flags, int P, int S, int L, int R, int E) {
+  o_mdata_t Cm = {C, ldc}, Am = {(double *)A, lda}, Bm = {(double *)B, ldb};
+  kern()->dmult_mm_blocked3(&Cm, &Am, &Bm, alpha, beta, flags, P, S, L, R, E, vpLen, nB, mB);
+}
+
+void o_kern_trsm(double *B, int ldb, const double *A, int lda, double alpha, int flags, int N,
+                 int S, int E) {
+  o_mdata_t Bm = {B, ldb}, Am = {(double *)A, lda};
+  kern()->dmmat_solve_blk(&Bm, &Am, alpha, flags, N, S, E, nB);
+}
+
+void o_kern_trsm_unb(double *B, int ldb, const double *A, int lda, double alpha, int flags, int N,
+                     int S, int E) {
+  o_mdata_t Bm = {B, ldb}, Am = {(double *)A, lda};
+  kern()->dmmat_solve_unb(&Bm, &Am, alpha, flags, N, S, E);
+}
+
+void o_kern_syrk(double *C, int ldc, const double *A, int lda, double alpha, double beta,
+                 int flags, int P, int S, int E) {
+  o_mdata_t Cm = {C, ldc}, Am = {(double *)A, lda};
+  kern()->dmmat_rank_blk(&Cm, &Am, alpha, beta, flags, P, S, E, vpLen, nB);
 }

@@ -47,6 +47,8 @@ typedef struct {
   double (*dvec_dot)(const o_mvec_t *X, const o_mvec_t *Y, double alpha, int N);
   void (*dvec_swap)(o_mvec_t *X, o_mvec_t *Y, int N);
   void (*dvec_invscal)(o_mvec_t *X, double alpha, int N);
+  void (*dmmat_solve_unb)(o_mdata_t *B, const o_mdata_t *A, double alpha, int flags, int N, int S,
+                          int E);
 } okern_t;
 
 /* calgo_port.c: the restated kernels */
@@ -78,6 +80,18 @@ int o_SolveLU(oview B, oview A, const int64_t *pivots, int npiv, int flags);
  * S = L^-T * Phi(L^T * Lbar) * L^-1, Phi = lower triangle, halved diagonal. */
 void o_chol_adjoint(int n, const double *L, int ldl, const double *Lbar, int ldb, double *Abar,
                     int lda);
+
+/* The kernel-level boundary itself (calgo/cmops.h:133-138, 203-214): the
+ * selected kernel set called with explicit block indices, as calgo.DMult /
+ * DSolveUnblk / DSolveBlk / DSymmRankBlk pass them (calgo/cmops.go:68-240). */
+void o_kern_gemm(double *C, int ldc, const double *A, int lda, const double *B, int ldb,
+                 double alpha, double beta, int flags, int P, int S, int L, int R, int E);
+void o_kern_trsm(double *B, int ldb, const double *A, int lda, double alpha, int flags, int N,
+                 int S, int E);
+void o_kern_trsm_unb(double *B, int ldb, const double *A, int lda, double alpha, int flags, int N,
+                     int S, int E);
+void o_kern_syrk(double *C, int ldc, const double *A, int lda, double alpha, double beta,
+                 int flags, int P, int S, int E);
 
 /* wall-clock helper (CLOCK_MONOTONIC, seconds) */
 double o_now(void);

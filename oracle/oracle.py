@@ -61,6 +61,10 @@ def lib():
             "o_SolveLU": (I, [V, V, P, I, I]),
             "o_chol_adjoint": (None, [I, P, I, P, I, P, I]),
             "o_now": (D, []),
+            "o_kern_gemm": (None, [P, I, P, I, P, I, D, D, I, I, I, I, I, I]),
+            "o_kern_trsm": (None, [P, I, P, I, D, I, I, I, I]),
+            "o_kern_trsm_unb": (None, [P, I, P, I, D, I, I, I, I]),
+            "o_kern_syrk": (None, [P, I, P, I, D, D, I, I, I, I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -140,6 +144,31 @@ def DecomposeLU(A, pivots, nb):
 def SolveLU(B, A, pivots, flags):
     assert pivots.dtype == np.int64
     return lib().o_SolveLU(_v(B), _v(A), pivots.ctypes.data, len(pivots), flags)
+
+
+def _ld(X):
+    if X.dtype != np.float64 or X.ndim != 2 or (X.shape[0] > 1 and X.strides[0] != 8):
+        raise TypeError("column-major float64 matrix expected")
+    return max(X.strides[1] // 8 if X.shape[1] > 1 else X.shape[0], 1)
+
+
+def kern_gemm(C, A, B, alpha, beta, flags, P, S, L, R, E):
+    """dmult_mm_blocked3 on C[R:E, S:L] with the selected kernels
+    (calgo/cmops.h:133-138). Arrays are the parents; indices as calgo's."""
+    lib().o_kern_gemm(C.ctypes.data, _ld(C), A.ctypes.data, _ld(A), B.ctypes.data, _ld(B),
+                      alpha, beta, flags, P, S, L, R, E)
+
+
+def kern_trsm(B, A, alpha, flags, N, S, E, unblocked=False):
+    """dmmat_solve_blk (or dmmat_solve_unb) on columns (LEFT) or rows
+    (RIGHT) S:E of B (calgo/cmops.h:203-209)."""
+    f = lib().o_kern_trsm_unb if unblocked else lib().o_kern_trsm
+    f(B.ctypes.data, _ld(B), A.ctypes.data, _ld(A), alpha, flags, N, S, E)
+
+
+def kern_syrk(C, A, alpha, beta, flags, P, S, E):
+    """dmmat_rank_blk on the block S:E (calgo/cmops.h:212-214)."""
+    lib().o_kern_syrk(C.ctypes.data, _ld(C), A.ctypes.data, _ld(A), alpha, beta, flags, P, S, E)
 
 
 def chol_adjoint(L, Lbar):

@@ -1,0 +1,123 @@
+"""GPU tests of failure reporting.
+
+* Bounded waits: with MOPS_SPIN_LIMIT=0 (read once per process) every
+  bounded wait gives up at once instead of polling, so the give-up path runs
+  without any kernel waiting. A split-k GEMM and a chained triangular solve
+  must then report the timeout: the host-memory calls fail with a runtime
+  error naming the wait, the device-resident ones through mops_d_status(),
+  which clears the word. Run in a child process so the setting stays there.
+* Refinement on a singular matrix: Inf/NaN in the solution must never read
+  as converged.
+"""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import matops_amd as mo
+from matops_amd import _lib
+from helpers import fmat
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CHILD = r"""
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+import numpy as np
+import torch
+import matops_amd as mo
+from matops_amd import _lib
+from helpers import fmat
+L = _lib.lib()
+# host-memory split-k GEMM (one 64x64 output tile, k = 8192: split in chunks)
+A, B = fmat(64, 8192, 1), fmat(8192, 64, 2)
+C = np.zeros((64, 64), order="F")
+try:
+    mo.Mult(C, A, B, 1.0, 0.0, mo.NOTRANS)
+    print("gemm-host: no error")
+except _lib.MopsError as e:
+    print("gemm-host:", e.code, "split-k" in e.msg)
+print("status-after-host:", L.mops_d_status(None))
+# device-resident split-k GEMM, then the status word
+dev = torch.device("cuda:0")
+tA = torch.from_numpy(A.T.copy()).to(dev).T
+tB = torch.from_numpy(B.T.copy()).to(dev).T
+tC = torch.zeros((64, 64), dtype=torch.float64, device=dev).T
+s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+rc = L.mops_d_dgemm(0, 64, 64, 8192, 1.0, tA.data_ptr(), 64, tB.data_ptr(), 8192, 0.0,
+                    tC.data_ptr(), 64, s)
+print("gemm-dev-rc:", rc)
+st = L.mops_d_status(s)
+print("status1:", st, "split-k" in _lib.last_error())
+print("status2:", L.mops_d_status(s))
+# chained triangular solve (LEFT, few right-hand sides)
+n = 512
+T = np.asfortranarray(np.tril(fmat(n, n, 3, "normal") * 0.1) + 3.0 * np.eye(n))
+X = fmat(n, 8, 4)
+try:
+    mo.SolveTrm(X, T, 1.0, mo.LEFT | mo.LOWER)
+    print("trsm-host: no error")
+except _lib.MopsError as e:
+    print("trsm-host:", e.code, "triangular solve" in e.msg)
+print("status-end:", L.mops_d_status(s))
+"""
+
+
+def test_bounded_waits_report_timeouts():
+    env = dict(os.environ, MOPS_SPIN_LIMIT="0")
+    print("[status] child with MOPS_SPIN_LIMIT=0", flush=True)
+    r = subprocess.run([sys.executable, "-c", CHILD, ROOT], env=env, capture_output=True,
+                       text=True, timeout=300)
+    out = r.stdout
+    assert r.returncode == 0, out + r.stderr
+    got = dict(line.split(":", 1) for line in out.strip().splitlines() if ":" in line)
+    assert got["gemm-host"].split() == ["-1000", "True"], out
+    assert got["status-after-host"].strip() == "0", out
+    assert got["gemm-dev-rc"].strip() == "0", out
+    assert got["status1"].split() == ["-1000", "True"], out
+    assert got["status2"].strip() == "0", out
+    assert got["trsm-host"].split() == ["-1000", "True"], out
+    assert got["status-end"].strip() == "0", out
+
+
+def test_status_clear_after_normal_calls():
+    A, B = fmat(64, 8192, 1), fmat(8192, 64, 2)
+    C = np.zeros((64, 64), order="F")
+    assert mo.Mult(C, A, B, 1.0, 0.0, mo.NOTRANS) is None
+    assert np.allclose(C, A @ B, rtol=0, atol=1e-10)
+    assert _lib.lib().mops_d_status(None) == 0
+
+
+def test_refinement_singular_matrix_not_converged():
+    n, nrhs = 300, 3
+    A = fmat(n, n, 7, "normal")
+    A[:, 17] = 0.0  # singular: a zero column
+    B = fmat(n, nrhs, 8, "normal")
+    for prec in (mo.PREC_FP64, mo.PREC_FP32):
+        X = np.zeros((n, nrhs), order="F")
+        rep = mo.SolveIR(X, A, B, prec=prec, tol=1e-12, maxit=3)
+        assert rep["converged"] == 0, (prec, rep)
+        assert not (rep["backward_error"] <= 1e-12), (prec, rep)
+
+
+def test_dmult_alpha_zero_propagates_inf_like_the_reference():
+    """alpha = 0 still forms the product (calgo/dmmat_gemm.c:297,413): an Inf
+    in A gives NaN in C; beta = 0 clears what C held."""
+    from oracle import oracle as O
+    m, n, k = 40, 30, 20
+    A, B = fmat(m, k, 1), fmat(k, n, 2)
+    A[5, 3] = np.inf
+    C = np.full((m, n), 7.0, order="F")
+    mo.Mult(C, A, B, 0.0, 0.0, mo.NOTRANS)
+    Co = np.full((m, n), 7.0, order="F")
+    O.use("reference" if O.reference_available() else "port")
+    O.Mult(Co, A, B, 0.0, 0.0, 0)
+    O.use("port")
+    np.testing.assert_array_equal(np.isnan(C), np.isnan(Co))
+    assert np.isnan(C[5]).all() and not np.isnan(np.delete(C, 5, axis=0)).any()
+    assert np.all(C[~np.isnan(C)] == 0.0)
