@@ -97,6 +97,20 @@ int mops_dmult_mm_blocked3(mops_mdata_t *C, const mops_mdata_t *A, const mops_md
 int mops_d_dgemm(int flags, int M, int N, int P, double alpha, const double *A, int lda,
                  const double *B, int ldb, double beta, double *C, int ldc, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results as mops_d_dgemm, but the
+ * scratch of the k split (partial tiles and flags) and the status word live
+ * in one static device array of 34 MiB per GPU (enough for every shape the
+ * split plan makes; a product needing more is rejected with ST_BADARG and a
+ * message), so the call allocates nothing. The array is shared by every
+ * call on that GPU and keeps the last call's contents: calls that overlap
+ * on one GPU race and must be serialized by the caller. A timed-out wait is
+ * reported by mops_d_status(). */
+int mops_d_dgemm_noalloc(int flags, int M, int N, int P, double alpha, const double *A, int lda,
+                         const double *B, int ldb, double beta, double *C, int ldc,
+                         void *stream);
+
 /* ---- TRSM: B = alpha*op(A)^-1*B (LEFT) or alpha*B*op(A)^-1 (RIGHT) ------- */
 
 /* Replaces dmmat_solve_blk (calgo/cmops.h:207-209, calgo/dmmat_solve.c:864),

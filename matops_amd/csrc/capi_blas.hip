@@ -74,7 +74,14 @@ void mops_thread_release(void) { thread_release(); }
 
 int mops_d_status(void* stream) {
   clear_error();
-  return take_status(as_stream(stream));
+  int st = take_status(as_stream(stream));
+  // the no-allocation GEMM keeps its status word in its own scratch
+  if (int* ns = noalloc_status_word()) {
+    StatusOverride so(ns);
+    const int st2 = take_status(as_stream(stream));
+    if (st == ST_OK) st = st2;
+  }
+  return st;
 }
 
 void mops_trace_enable(int on) { trace_enable(on); }

@@ -110,6 +110,9 @@ int take_status(hipStream_t s);
 // or the pointer installed by a StatusOverride in scope (entry points that
 // may not allocate pass one inside their own scratch).
 int* active_status();
+// Status word of the no-allocation GEMM on the current device (inside its
+// scratch array), nullptr until that entry point first ran there.
+int* noalloc_status_word();
 class StatusOverride {
  public:
   explicit StatusOverride(int* p);
