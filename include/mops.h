@@ -122,7 +122,7 @@ int mops_d_dgemm_noalloc(int flags, int M, int N, int P, double alpha, const dou
 int mops_dmmat_solve_blk(mops_mdata_t *B, const mops_mdata_t *A, double alpha, int flags, int N,
                          int S, int E, int NB);
 
-/* Replaces dmmat_solve_unb (calgo/cmops.h:203-204, calgo/dmmat_solve.c:400),
+/* Replaces dmmat_solve_unb (calgo/cmops.h:202-204, calgo/dmmat_solve.c:400),
  * the unblocked form of the same solve: identical arguments and results
  * except for how the triangle is chosen, which follows dmmat_solve_unb:
  * LOWER selects the lower triangle, otherwise the upper one (the blocked
@@ -153,6 +153,68 @@ int mops_dmmat_rank_blk(mops_mdata_t *C, const mops_mdata_t *A, double alpha, do
 /* Device-resident SYRK: C is N x N, A is N x P (P x N with TRANSA). */
 int mops_d_dsyrk(int flags, int N, int P, double alpha, const double *A, int lda, double beta,
                  double *C, int ldc, void *stream);
+
+/* ---- the other BLAS-3 kernels -------------------------------------------- */
+
+/* Replaces dmult_symm_blocked2 (calgo/cmops.h:148-152, calgo/dmmat_symm.c:253),
+ * bound by calgo.DMultSymm (calgo/cmops.go:95-118) under matops.MultSym
+ * (mmult.go:266-326). A holds one triangle (UPPER, else lower) of a P x P
+ * symmetric matrix S. LEFT: C[R:E, S:L] = beta C + alpha S[R:E, :] B[:, S:L];
+ * otherwise (RIGHT): C[R:E, S:L] = beta C + alpha B[R:E, :] S[:, S:L].
+ * Differences from calgo, which only arise off its tested forms (LEFT or
+ * RIGHT with one triangle bit, no TRANSB; calgo/tmult_test.go:347-455): the
+ * product is the symmetric one whatever other bits are set (calgo's RIGHT
+ * branch reads B at inconsistent offsets under TRANSB, and its diagonal
+ * blocks take the other triangle when no triangle bit is given). beta == 0
+ * writes zeros. Host memory, synchronous. */
+int mops_dmult_symm_blocked2(mops_mdata_t *C, const mops_mdata_t *A, const mops_mdata_t *B,
+                             double alpha, double beta, int flags, int P, int S, int L, int R,
+                             int E, int vlen, int NB, int MB);
+/* Device-resident SYMM: C (M x N); A M x M (LEFT) or N x N (RIGHT). */
+int mops_d_dsymm(int flags, int M, int N, double alpha, const double *A, int lda, const double *B,
+                 int ldb, double beta, double *C, int ldc, void *stream);
+
+/* Replaces dmmat_trmm_blk (calgo/cmops.h:199-200, calgo/dmmat_trid.c:780),
+ * bound by calgo.DTrmmBlk (calgo/cmops.go:146-166) under matops.MultTrm
+ * (mmult.go:338-372): B = alpha op(A) B on columns S:E of B (LEFT, no RIGHT
+ * bit) or B = alpha B op(A) on rows S:E (RIGHT); A N x N triangular (UPPER,
+ * else lower), TRANSA transposes, UNIT never reads the diagonal. NB is a
+ * CPU hint, ignored. Host memory, synchronous. */
+int mops_dmmat_trmm_blk(mops_mdata_t *B, const mops_mdata_t *A, double alpha, int flags, int N,
+                        int S, int E, int NB);
+/* Device-resident TRMM: B N x nrhs (LEFT) or nrhs x N (RIGHT). */
+int mops_d_dtrmm(int flags, int N, int nrhs, double alpha, const double *A, int lda, double *B,
+                 int ldb, void *stream);
+
+/* Replaces dmmat_rank2_blk (calgo/cmops.h:216-219, calgo/dmmat_rank.c:132),
+ * bound by calgo.DSymmRank2Blk (calgo/cmops.go:244-268) under
+ * matops.RankUpdate2Sym (mmult.go:453-476): the LOWER (else UPPER) triangle
+ * of C[S:E, S:E] = beta C + alpha (A B^T + B A^T) with rows S:E of A and B,
+ * or alpha (A^T B + B^T A) with columns S:E under TRANSA; inner length P.
+ * BLAS semantics; calgo differs for beta != 1 (it scales only the diagonal
+ * blocks, and twice without TRANSA), for ldB != ldA (it steps B with A's
+ * leading dimension) and, for LOWER, with S > 0 (as dmmat_rank_blk).
+ * Host memory, synchronous. */
+int mops_dmmat_rank2_blk(mops_mdata_t *C, const mops_mdata_t *A, const mops_mdata_t *B,
+                         double alpha, double beta, int flags, int P, int S, int E, int vlen,
+                         int NB);
+/* Device-resident SYR2K: C N x N, A and B N x P (P x N with TRANSA). */
+int mops_d_dsyr2k(int flags, int N, int P, double alpha, const double *A, int lda,
+                  const double *B, int ldb, double beta, double *C, int ldc, void *stream);
+
+/* Replaces dmmat_trmupd_blk (calgo/cmops.h:222-226, calgo/dmmat_rank.c:292),
+ * bound by calgo.DTrmUpdBlk (calgo/cmops.go:273-297) under matops.UpdateTrm
+ * (mmult.go:501-520): the UPPER (else lower) triangle of C[S:E, S:E] =
+ * beta C + alpha op(A)[S:E, :] op(B)[:, S:E], op per TRANSA / TRANSB, inner
+ * length P. BLAS semantics; calgo scales only its diagonal blocks by beta
+ * and, for UPPER with S > 0, indexes the rows above them from row 0.
+ * Host memory, synchronous. */
+int mops_dmmat_trmupd_blk(mops_mdata_t *C, const mops_mdata_t *A, const mops_mdata_t *B,
+                          double alpha, double beta, int flags, int P, int S, int E, int vlen,
+                          int NB);
+/* Device-resident triangular update: C N x N, op(A) N x P, op(B) P x N. */
+int mops_d_dtrmupd(int flags, int N, int P, double alpha, const double *A, int lda,
+                   const double *B, int ldb, double beta, double *C, int ldc, void *stream);
 
 /* ---- Cholesky ------------------------------------------------------------ */
 

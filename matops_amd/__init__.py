@@ -8,5 +8,5 @@ from .matops import (  # noqa: F401
     LEFT, LOWER, NONE, NOTRANS, RIGHT, TRANS, TRANSA, TRANSB, UNIT, UPPER,
     BlockingParams, CholeskyAdjoint, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
     PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
-    blockIndex4, divideWork, scheduleWork,
+    blockIndex4, divideWork, scheduleWork, MultSym, MultTrm, RankUpdate2Sym, UpdateTrm,
 )

@@ -45,6 +45,15 @@ SIGNATURES = {
     "mops_d_dtrsm": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _P]),
     "mops_dmmat_rank_blk": (_I, [_PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I]),
     "mops_d_dsyrk": (_I, [_I, _I, _I, _D, _P, _I, _D, _P, _I, _P]),
+    "mops_dmult_symm_blocked2": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I, _I, _I,
+                                      _I]),
+    "mops_d_dsymm": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
+    "mops_dmmat_trmm_blk": (_I, [_PM, _PM, _D, _I, _I, _I, _I, _I]),
+    "mops_d_dtrmm": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _P]),
+    "mops_dmmat_rank2_blk": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I]),
+    "mops_d_dsyr2k": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
+    "mops_dmmat_trmupd_blk": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I]),
+    "mops_d_dtrmupd": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
     "mops_dpotrf": (_I, [_PM, _I, _I, _I]),
     "mops_dpotrs": (_I, [_PM, _PM, _I, _I, _I]),
     "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),

@@ -40,6 +40,28 @@ int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double
 int trsm_pair_d(hipStream_t s, int flags1, int flags2, int N, int nrhs, const double* A,
                 long lda, double* B, long ldb);
 
+// The other BLAS-3 operations (blas3x.hip). symm_d: C (M x N) = beta C +
+// alpha S B (left: S = sym(A), M x M) or alpha B S (S N x N), sym(A) the
+// symmetric matrix whose lower (else upper) triangle A stores; needs
+// symm_ws_bytes(order of S). trmm_d: B = alpha op(A) B (LEFT, B N x nrhs) or
+// alpha B op(A) (RIGHT, B nrhs x N), triangle and UNIT as in trsm_d; needs
+// trmm_ws_bytes. syr2k_d: one triangle (LOWER, else UPPER) of C (N x N) =
+// beta C + alpha (op(A) op(B)^T + op(B) op(A)^T), op = transpose with
+// TRANSA, inner length P. trmupd_d: one triangle (UPPER, else LOWER) of
+// C = beta C + alpha op(A) op(B) (TRANSA / TRANSB).
+size_t symm_ws_bytes(int n);
+int sym_expand_d(hipStream_t s, bool lower, int n, const double* A, long lda, double* Out,
+                 long ldo);
+int symm_d(hipStream_t s, bool left, bool lower, int M, int N, double alpha, const double* A,
+           long lda, const double* B, long ldb, double beta, double* C, long ldc);
+size_t trmm_ws_bytes(int N, int nrhs);
+int trmm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
+           double* B, long ldb);
+int syr2k_d(hipStream_t s, int flags, int N, int P, double alpha, const double* A, long lda,
+            const double* B, long ldb, double beta, double* C, long ldc);
+int trmupd_d(hipStream_t s, int flags, int N, int P, double alpha, const double* A, long lda,
+             const double* B, long ldb, double beta, double* C, long ldc);
+
 // Cholesky of the N x N matrix A in place (LOWER or UPPER flag), outer block
 // nb. `info` (device int, may be null) receives the 0-based global index of
 // the first NaN square root, or stays -1. `badval` (device double, may be

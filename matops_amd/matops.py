@@ -177,6 +177,112 @@ def Mult(C, A, B, alpha, beta, flags):
     return None
 
 
+def MultSym(C, A, B, alpha, beta, flags):
+    """C = beta*C + alpha*A*B (LEFT) or alpha*B*A (RIGHT) with A symmetric,
+    one triangle stored (mmult.go:266-326). Validation as the reference's
+    (it checks the shapes of C = A*B for every flag combination)."""
+    ar, ac = A.shape
+    br, bc = B.shape
+    cr, cc = C.shape
+    tt = flags & (TRANSA | TRANSB)
+    if tt == TRANSA | TRANSB:
+        empty = ac == 0 or br == 0
+        ok = ar == ac and cr == ac and cc == br and ar == bc
+    elif tt == TRANSA:
+        empty = ac == 0 or bc == 0
+        ok = ar == ac and cr == ac and cc == bc and ar == br
+    elif tt == TRANSB:
+        empty = ar == 0 or br == 0
+        ok = ar == ac and cr == ar and cc == br and ac == bc
+    else:
+        empty = ar == 0 or bc == 0
+        ok = ar == ac and cr == ar and cc == bc and ac == br
+    if empty:
+        return None
+    if not ok:
+        return MatopsError("MultSym: size mismatch")
+    psize = C.size * A.shape[1]
+    Cm, _, _ = _view(C)
+    Am, _, _ = _view(A)
+    Bm, _, _ = _view(B)
+    L = _lib.lib()
+
+    def worker(cs, ce, rs, re_):
+        check("mops_dmult_symm_blocked2", L.mops_dmult_symm_blocked2(
+            ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags,
+            ac, cs, ce, rs, re_, _vpLen, _nB, _mB))
+
+    if _nWorker <= 1 or psize <= _limitOne:
+        worker(0, cc, 0, cr)
+        return None
+    colworks, rowworks = divideWork(cr, cc, _nWorker)
+    scheduleWork(colworks, rowworks, cc, cr, worker)
+    return None
+
+
+def MultTrm(B, A, alpha, flags):
+    """B = alpha*op(A)*B (LEFT) or alpha*B*op(A) (RIGHT), A triangular
+    (mmult.go:338-372)."""
+    br, bc = B.shape
+    ar, ac = A.shape
+    empty, ok = False, True
+    side = flags & (LEFT | RIGHT)
+    if side == LEFT:
+        empty = br == 0 or bc == 0
+        ok = br == ac and ac == ar
+    elif side == RIGHT:
+        empty = bc == 0 or br == 0
+        ok = bc == ar and ac == ar
+    if empty:
+        return None
+    if not ok:
+        return _onError("A, B size mismatch")
+    E = br if flags & RIGHT else bc
+    if ac == 0 or E <= 0:  # calgo/dmmat_trid.c:790-791
+        return None
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dmmat_trmm_blk", _lib.lib().mops_dmmat_trmm_blk(
+        ctypes.byref(Bm), ctypes.byref(Am), alpha, flags, ac, 0, E, _nB))
+    return None
+
+
+def RankUpdate2Sym(C, A, B, alpha, beta, flags):
+    """C = beta*C + alpha*A*B^T + alpha*B*A^T (alpha*A^T*B + alpha*B^T*A
+    with TRANSA) on one triangle (mmult.go:453-476)."""
+    if C.shape[0] != C.shape[1]:
+        return _onError("C not a square matrix")
+    P = A.shape[0] if flags & TRANSA else A.shape[1]
+    E = C.shape[0]
+    if P == 0 or E == 0:
+        return None
+    Cm, _, _ = _view(C)
+    Am, _, _ = _view(A)
+    Bm, _, _ = _view(B)
+    check("mops_dmmat_rank2_blk", _lib.lib().mops_dmmat_rank2_blk(
+        ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags, P, 0, E,
+        _vpLen, _nB))
+    return None
+
+
+def UpdateTrm(C, A, B, alpha, beta, flags):
+    """C = beta*C + alpha*op(A)*op(B) on the LOWER or UPPER triangle of C
+    (mmult.go:501-520)."""
+    if C.shape[0] != C.shape[1]:
+        return _onError("C not a square matrix")
+    P = A.shape[0] if flags & TRANSA else A.shape[1]
+    E = C.shape[0]
+    if P == 0 or E == 0:
+        return None
+    Cm, _, _ = _view(C)
+    Am, _, _ = _view(A)
+    Bm, _, _ = _view(B)
+    check("mops_dmmat_trmupd_blk", _lib.lib().mops_dmmat_trmupd_blk(
+        ctypes.byref(Cm), ctypes.byref(Am), ctypes.byref(Bm), alpha, beta, flags, P, 0, E,
+        _vpLen, _nB))
+    return None
+
+
 def SolveTrm(B, A, alpha, flags):
     """B = alpha*op(A)^-1*B or alpha*B*op(A)^-1  (mmult.go:388-422)."""
     br, bc = B.shape

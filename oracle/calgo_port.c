@@ -10,6 +10,7 @@
  * calgo/inner_ddot.h:13.
  */
 #include <stddef.h>
+#include <stdlib.h>
 
 #include "oracle.h"
 
@@ -315,8 +316,186 @@ static void p_dmmat_solve_unb(o_mdata_t *B, const o_mdata_t *A, double alpha, in
   p_dmmat_solve_blk(B, A, 1.0, f, N, S, E, 0);
 }
 
+/* dmult_symm_blocked2, calgo/dmmat_symm.c:253-390: C[R:E, S:L] = beta C +
+ * alpha S B (LEFT) or alpha B S (otherwise), S the symmetric matrix whose
+ * UPPER (else lower) triangle A holds (the off-diagonal blocks pick their
+ * offsets by UPPER, :316 and :366); beta applied once per MB x NB block of
+ * C (dscale_tile, :313 and :363), then the three block products added. */
+static inline double symA(const o_mdata_t *A, int upper, int i, int k) {
+  const int stored = upper ? i <= k : i >= k;
+  return stored ? A->md[i + (ptrdiff_t)k * A->step] : A->md[k + (ptrdiff_t)i * A->step];
+}
+static void p_dmult_symm_blocked2(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B,
+                                  double alpha, double beta, int flags, int P, int S, int L, int R,
+                                  int E, int vlen, int NB, int MB) {
+  (void)vlen;
+  (void)NB;
+  (void)MB;
+  if (L - S <= 0 || E - R <= 0) return;
+  const int upper = (flags & O_UPPER) != 0;
+  p_scale_tile(&C->md[(ptrdiff_t)S * C->step + R], C->step, beta, E - R, L - S);
+  for (int j = S; j < L; j++)
+    for (int i = R; i < E; i++) {
+      double c0 = 0.0, c1 = 0.0;
+      int k = 0;
+      for (; k + 1 < P; k += 2) {
+        if (flags & O_LEFT) {
+          c0 += symA(A, upper, i, k) * B->md[k + (ptrdiff_t)j * B->step];
+          c1 += symA(A, upper, i, k + 1) * B->md[k + 1 + (ptrdiff_t)j * B->step];
+        } else {
+          c0 += B->md[i + (ptrdiff_t)k * B->step] * symA(A, upper, k, j);
+          c1 += B->md[i + (ptrdiff_t)(k + 1) * B->step] * symA(A, upper, k + 1, j);
+        }
+      }
+      if (k < P)
+        c0 += (flags & O_LEFT) ? symA(A, upper, i, k) * B->md[k + (ptrdiff_t)j * B->step]
+                               : B->md[i + (ptrdiff_t)k * B->step] * symA(A, upper, k, j);
+      C->md[i + (ptrdiff_t)j * C->step] += alpha * (c0 + c1);
+    }
+}
+
+/* dmmat_trmm_blk, calgo/dmmat_trid.c:780-827: B = alpha op(A) B on columns
+ * S:E (LEFT) or B = alpha B op(A) on rows S:E (RIGHT); triangle UPPER, else
+ * lower (:799, :815); UNIT never reads the diagonal. Restated as the
+ * product with the triangle, each output from the inputs before the
+ * update. */
+static void p_dmmat_trmm_blk(o_mdata_t *B, const o_mdata_t *A, double alpha, int flags, int N,
+                             int S, int E, int NB) {
+  (void)NB;
+  if (E - S <= 0 || N <= 0) return;
+  const int upper = (flags & O_UPPER) != 0;
+  const int unit = (flags & O_UNIT) != 0;
+  double *x = malloc(sizeof(double) * (size_t)N);
+  for (int v = S; v < E; v++) {
+    /* one column (LEFT) or row (RIGHT) of B at a time */
+    const ptrdiff_t base = flags & O_RIGHT ? v : (ptrdiff_t)v * B->step;
+    const ptrdiff_t inc = flags & O_RIGHT ? B->step : 1;
+    for (int i = 0; i < N; i++) x[i] = B->md[base + i * inc];
+    for (int i = 0; i < N; i++) {
+      double c = 0.0;
+      for (int k = 0; k < N; k++) {
+        /* LEFT: out[i] = sum_k op(A)[i][k] x[k]; RIGHT: out[i] = sum_k x[k] op(A)[k][i] */
+        const int r = flags & O_RIGHT ? k : i, q = flags & O_RIGHT ? i : k;
+        const int ar = flags & O_TRANSA ? q : r, ac = flags & O_TRANSA ? r : q;
+        if (upper ? ar > ac : ar < ac) continue;
+        const double a = (ar == ac && unit) ? 1.0 : A->md[ar + (ptrdiff_t)ac * A->step];
+        c += a * x[k];
+      }
+      B->md[base + i * inc] = alpha * c;
+    }
+  }
+  free(x);
+}
+
+/* dmmat_rank2_blk, calgo/dmmat_rank.c:132-239: for each NB block i of
+ * [S, E): the diagonal triangle gets beta and the two products
+ * (_dmmat_rank_diag, :18-58; without TRANSA beta is applied in both halves,
+ * :213 and :226, with TRANSA only in the first, :184), the rectangle left
+ * of it (LOWER: columns 0..i, rows S.. of the second operand) or right of
+ * it (UPPER) the two products only. B is stepped with A's leading
+ * dimension (Bd.step = A->step, :160), as the reference does. */
+static double r2dot(const o_mdata_t *X, const o_mdata_t *Y, int trans, int i, int j, int P) {
+  double c0 = 0.0, c1 = 0.0;
+  int k = 0;
+  for (; k + 1 < P; k += 2) {
+    if (trans) {
+      c0 += X->md[k + (ptrdiff_t)i * X->step] * Y->md[k + (ptrdiff_t)j * Y->step];
+      c1 += X->md[k + 1 + (ptrdiff_t)i * X->step] * Y->md[k + 1 + (ptrdiff_t)j * Y->step];
+    } else {
+      c0 += X->md[i + (ptrdiff_t)k * X->step] * Y->md[j + (ptrdiff_t)k * Y->step];
+      c1 += X->md[i + (ptrdiff_t)(k + 1) * X->step] * Y->md[j + (ptrdiff_t)(k + 1) * Y->step];
+    }
+  }
+  if (k < P)
+    c0 += trans ? X->md[k + (ptrdiff_t)i * X->step] * Y->md[k + (ptrdiff_t)j * Y->step]
+                : X->md[i + (ptrdiff_t)k * X->step] * Y->md[j + (ptrdiff_t)k * Y->step];
+  return c0 + c1;
+}
+static void p_dmmat_rank2_blk(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B, double alpha,
+                              double beta, int flags, int P, int S, int E, int vlen, int NB) {
+  (void)vlen;
+  if (E - S <= 0 || P <= 0) return;
+  if (NB > MAX_VP_COLS || NB <= 0) NB = MAX_VP_COLS;
+  if (NB > E - S) NB = E - S;
+  const int trans = (flags & O_TRANSA) != 0;
+  const int lower = (flags & O_LOWER) != 0;
+  o_mdata_t Bs = {B->md, A->step};
+  const double beta2 = trans ? 1.0 : beta;
+  for (int half = 0; half < 2; half++) {
+    const o_mdata_t *X = half ? &Bs : A, *Y = half ? A : &Bs;
+    for (int i = S; i < E; i += NB) {
+      const int nI = E - i < NB ? E - i : NB;
+      for (int r = 0; r < nI; r++) {
+        const int row = i + r;
+        const int c0 = !(flags & O_UPPER) ? i : row, c1 = !(flags & O_UPPER) ? row : i + nI - 1;
+        for (int c = c0; c <= c1; c++) {
+          double *cp = &C->md[row + (ptrdiff_t)c * C->step];
+          const double b = half ? beta2 : beta;
+          *cp = b == 0.0 ? 0.0 : (b == 1.0 ? *cp : b * *cp);
+          *cp += alpha * r2dot(X, Y, trans, row, c, P);
+        }
+      }
+      if (lower) {
+        for (int c = 0; c < i; c++)
+          for (int row = i; row < i + nI; row++)
+            C->md[row + (ptrdiff_t)c * C->step] += alpha * r2dot(X, Y, trans, row, S + c, P);
+      } else {
+        for (int c = i + nI; c < E; c++)
+          for (int row = i; row < i + nI; row++)
+            C->md[row + (ptrdiff_t)c * C->step] += alpha * r2dot(X, Y, trans, row, c, P);
+      }
+    }
+  }
+}
+
+/* dmmat_trmupd_blk, calgo/dmmat_rank.c:292-360: per NB block i of [S, E),
+ * the diagonal triangle is scaled by beta and updated row by row
+ * (_dmmat_trmupd_diag, :247-290); for UPPER the rows above it (rows 0..i
+ * of C with the rows of op(A) from S, :320-324; the reference's offset
+ * &A[S] equals that only without TRANSA or with S = 0), for LOWER the rows
+ * below it (:349-354) get alpha op(A) op(B) without beta. */
+static void p_dmmat_trmupd_blk(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B,
+                               double alpha, double beta, int flags, int P, int S, int E, int vlen,
+                               int NB) {
+  (void)vlen;
+  if (E - S <= 0 || P <= 0) return;
+  if (NB > MAX_VP_COLS || NB <= 0) NB = MAX_VP_COLS;
+  if (NB > E - S) NB = E - S;
+  const int upper = (flags & O_UPPER) != 0;
+  for (int i = S; i < E; i += NB) {
+    const int nI = E - i < NB ? E - i : NB;
+    /* diagonal triangle */
+    for (int r = 0; r < nI; r++) {
+      const int row = i + r;
+      const int c0 = upper ? row : i, c1 = upper ? i + nI - 1 : row;
+      for (int c = c0; c <= c1; c++) {
+        double *cp = &C->md[row + (ptrdiff_t)c * C->step];
+        *cp = beta == 0.0 ? 0.0 : (beta == 1.0 ? *cp : beta * *cp);
+        double c0s = 0.0, c1s = 0.0;
+        int k = 0;
+        for (; k + 1 < P; k += 2) {
+          c0s += opA(A, flags, row, k) * opB(B, flags, k, c);
+          c1s += opA(A, flags, row, k + 1) * opB(B, flags, k + 1, c);
+        }
+        if (k < P) c0s += opA(A, flags, row, k) * opB(B, flags, k, c);
+        *cp += alpha * (c0s + c1s);
+      }
+    }
+    /* off-diagonal rectangle: rows [r0, r1) of C, columns [i, i + nI) */
+    const int r0 = upper ? 0 : i + nI, r1 = upper ? i : E;
+    for (int c = i; c < i + nI; c++)
+      for (int row = r0; row < r1; row++) {
+        const int arow = upper ? S + row : row;
+        double c0s = 0.0;
+        for (int k = 0; k < P; k++) c0s += opA(A, flags, arow, k) * opB(B, flags, k, c);
+        C->md[row + (ptrdiff_t)c * C->step] += alpha * c0s;
+      }
+  }
+}
+
 const okern_t o_port_kernels = {
     p_dmult_mm_blocked3, p_dmmat_solve_blk,    p_dmmat_rank_blk, p_dmvec_symv_rank,
     p_dmvec_solve_blocked, p_dmult_gemv_blocked, p_dvec_dot,       p_dvec_swap,
-    p_dvec_invscal,      p_dmmat_solve_unb,
+    p_dvec_invscal,      p_dmmat_solve_unb,    p_dmult_symm_blocked2, p_dmmat_trmm_blk,
+    p_dmmat_rank2_blk,   p_dmmat_trmupd_blk,
 };

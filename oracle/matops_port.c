@@ -54,6 +54,10 @@ int o_use_kernels(int which, const char *path) {
   SYM(dvec_swap)
   SYM(dvec_invscal)
   SYM(dmmat_solve_unb)
+  SYM(dmult_symm_blocked2)
+  SYM(dmmat_trmm_blk)
+  SYM(dmmat_rank2_blk)
+  SYM(dmmat_trmupd_blk)
 #undef SYM
   K = k;
   K_init = 1;
@@ -67,7 +71,8 @@ int o_kernels_in_use(void) { return K_which; }
  * from the reference (0 dmult_mm_blocked3, 1 dmmat_solve_blk, 2
  * dmmat_rank_blk, 3 dmvec_symv_rank, 4 dmvec_solve_blocked, 5
  * dmult_gemv_blocked, 6 dvec_dot, 7 dvec_swap, 8 dvec_invscal, 9
- * dmmat_solve_unb). Used to
+ * dmmat_solve_unb, 10 dmult_symm_blocked2, 11 dmmat_trmm_blk, 12
+ * dmmat_rank2_blk, 13 dmmat_trmupd_blk). Used to
  * localise differences between the restatement and the reference. */
 int o_use_kernel_mask(int mask, const char *path) {
   if (o_use_kernels(1, path) != 0) return -1;
@@ -219,6 +224,111 @@ int o_Mult(oview C, oview A, oview B, double alpha, double beta, int flags) {
   for (int i = 0; i < n; i++) pthread_join(th[i], NULL);
   free(th);
   free(tk);
+  return 0;
+}
+
+/* ---- MultSym, mmult.go:266-326 -> calgo.DMultSymm, calgo/cmops.go:95-118 ---- */
+typedef struct {
+  o_mdata_t C, A, B;
+  double alpha, beta;
+  int flags, P, S, L, R, E;
+} symm_task;
+
+static void *symm_worker(void *arg) {
+  symm_task *t = (symm_task *)arg;
+  kern()->dmult_symm_blocked2(&t->C, &t->A, &t->B, t->alpha, t->beta, t->flags, t->P, t->S, t->L,
+                              t->R, t->E, vpLen, nB, mB);
+  return NULL;
+}
+
+int o_MultSym(oview C, oview A, oview B, double alpha, double beta, int flags) {
+  int ar = A.rows, ac = A.cols, br = B.rows, bc = B.cols, cr = C.rows, cc = C.cols;
+  int empty, ok;
+  switch (flags & (O_TRANSA | O_TRANSB)) {
+    case O_TRANSA | O_TRANSB:
+      empty = ac == 0 || br == 0;
+      ok = ar == ac && cr == ac && cc == br && ar == bc;
+      break;
+    case O_TRANSA:
+      empty = ac == 0 || bc == 0;
+      ok = ar == ac && cr == ac && cc == bc && ar == br;
+      break;
+    case O_TRANSB:
+      empty = ar == 0 || br == 0;
+      ok = ar == ac && cr == ar && cc == br && ac == bc;
+      break;
+    default:
+      empty = ar == 0 || bc == 0;
+      ok = ar == ac && cr == ar && cc == bc && ac == br;
+  }
+  if (empty) return 0;
+  if (!ok) return 1; /* "MultSym: size mismatch" */
+  int64_t psize = (int64_t)numel(C) * (int64_t)A.cols;
+  o_mdata_t Cm = md(C), Am = md(A), Bm = md(B);
+  if (nWorker <= 1 || psize <= limitOne) {
+    kern()->dmult_symm_blocked2(&Cm, &Am, &Bm, alpha, beta, flags, A.cols, 0, C.cols, 0, C.rows,
+                                vpLen, nB, mB);
+    return 0;
+  }
+  int colworks, rowworks;
+  divideWork(C.rows, C.cols, nWorker, &colworks, &rowworks);
+  int ntask = colworks * rowworks;
+  pthread_t *th = malloc(sizeof(pthread_t) * ntask);
+  symm_task *tk = malloc(sizeof(symm_task) * ntask);
+  int n = 0;
+  for (int k = 0; k < colworks; k++) {
+    int cs = blockIndex4(k, colworks, C.cols), ce = blockIndex4(k + 1, colworks, C.cols);
+    for (int l = 0; l < rowworks; l++) {
+      int rs = blockIndex4(l, rowworks, C.rows), re = blockIndex4(l + 1, rowworks, C.rows);
+      symm_task t = {Cm, Am, Bm, alpha, beta, flags, A.cols, cs, ce, rs, re};
+      tk[n] = t;
+      pthread_create(&th[n], NULL, symm_worker, &tk[n]);
+      n++;
+    }
+  }
+  for (int i = 0; i < n; i++) pthread_join(th[i], NULL);
+  free(th);
+  free(tk);
+  return 0;
+}
+
+/* ---- MultTrm, mmult.go:338-372 -> calgo.DTrmmBlk, calgo/cmops.go:146-166 ---- */
+int o_MultTrm(oview B, oview A, double alpha, int flags) {
+  int ok = 1, empty = 0;
+  int br = B.rows, bc = B.cols, ar = A.rows, ac = A.cols;
+  switch (flags & (O_LEFT | O_RIGHT)) {
+    case O_LEFT:
+      empty = br == 0 || bc == 0;
+      ok = br == ac && ac == ar;
+      break;
+    case O_RIGHT:
+      empty = bc == 0 || br == 0;
+      ok = bc == ar && ac == ar;
+      break;
+  }
+  if (empty) return 0;
+  if (!ok) return 1; /* "A, B size mismatch" */
+  int E = flags & O_RIGHT ? br : bc;
+  o_mdata_t Bm = md(B), Am = md(A);
+  kern()->dmmat_trmm_blk(&Bm, &Am, alpha, flags, ac, 0, E, nB);
+  return 0;
+}
+
+/* ---- RankUpdate2Sym, mmult.go:453-476 -> DSymmRank2Blk, cmops.go:244-268 ---- */
+int o_RankUpdate2Sym(oview C, oview A, oview B, double alpha, double beta, int flags) {
+  if (C.rows != C.cols) return 1; /* "C not a square matrix" */
+  int P = flags & O_TRANSA ? A.rows : A.cols;
+  o_mdata_t Cm = md(C), Am = md(A), Bm = md(B);
+  kern()->dmmat_rank2_blk(&Cm, &Am, &Bm, alpha, beta, flags, P, 0, C.rows, vpLen, nB);
+  return 0;
+}
+
+/* ---- UpdateTrm, mmult.go:501-520 -> DTrmUpdBlk, cmops.go:273-297 ---- */
+int o_UpdateTrm(oview C, oview A, oview B, double alpha, double beta, int flags) {
+  if (C.rows != C.cols) return 1; /* "C not a square matrix" */
+  int P = flags & O_TRANSA ? A.rows : A.cols;
+  o_mdata_t Cm = md(C), Am = md(A), Bm = md(B);
+  kern()->dmmat_trmupd_blk(&Cm, &Am, &Bm, alpha, beta, flags, P, 0, C.rows, vpLen, nB);
   return 0;
 }
 

@@ -49,6 +49,15 @@ typedef struct {
   void (*dvec_invscal)(o_mvec_t *X, double alpha, int N);
   void (*dmmat_solve_unb)(o_mdata_t *B, const o_mdata_t *A, double alpha, int flags, int N, int S,
                           int E);
+  void (*dmult_symm_blocked2)(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B, double alpha,
+                              double beta, int flags, int P, int S, int L, int R, int E, int vlen,
+                              int NB, int MB);
+  void (*dmmat_trmm_blk)(o_mdata_t *B, const o_mdata_t *A, double alpha, int flags, int N, int S,
+                         int E, int NB);
+  void (*dmmat_rank2_blk)(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B, double alpha,
+                          double beta, int flags, int P, int S, int E, int vlen, int NB);
+  void (*dmmat_trmupd_blk)(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B, double alpha,
+                           double beta, int flags, int P, int S, int E, int vlen, int NB);
 } okern_t;
 
 /* calgo_port.c: the restated kernels */
@@ -73,6 +82,10 @@ int o_DecomposeCHOL(oview A, int flags, int nb, int *panic_index, double *panic_
 void o_SolveCHOL(oview B, oview A, int flags);
 int o_DecomposeLU(oview A, int64_t *pivots, int npiv, int nb);
 int o_SolveLU(oview B, oview A, const int64_t *pivots, int npiv, int flags);
+int o_MultSym(oview C, oview A, oview B, double alpha, double beta, int flags);
+int o_MultTrm(oview B, oview A, double alpha, int flags);
+int o_RankUpdate2Sym(oview C, oview A, oview B, double alpha, double beta, int flags);
+int o_UpdateTrm(oview C, oview A, oview B, double alpha, double beta, int flags);
 
 /* Reference adjoint of the Cholesky factorization (closed form, built on
  * the oracle's TRSM/GEMM): given the lower factor L and Lbar (lower),

@@ -59,6 +59,10 @@ def lib():
             "o_SolveCHOL": (None, [V, V, I]),
             "o_DecomposeLU": (I, [V, P, I, I]),
             "o_SolveLU": (I, [V, V, P, I, I]),
+            "o_MultSym": (I, [V, V, V, D, D, I]),
+            "o_MultTrm": (I, [V, V, D, I]),
+            "o_RankUpdate2Sym": (I, [V, V, V, D, D, I]),
+            "o_UpdateTrm": (I, [V, V, V, D, D, I]),
             "o_chol_adjoint": (None, [I, P, I, P, I, P, I]),
             "o_now": (D, []),
             "o_kern_gemm": (None, [P, I, P, I, P, I, D, D, I, I, I, I, I, I]),
@@ -121,6 +125,22 @@ def SolveTrm(B, A, alpha, flags):
 
 def RankUpdateSym(C, A, alpha, beta, flags):
     return lib().o_RankUpdateSym(_v(C), _v(A), alpha, beta, flags)
+
+
+def MultSym(C, A, B, alpha, beta, flags):
+    return lib().o_MultSym(_v(C), _v(A), _v(B), alpha, beta, flags)
+
+
+def MultTrm(B, A, alpha, flags):
+    return lib().o_MultTrm(_v(B), _v(A), alpha, flags)
+
+
+def RankUpdate2Sym(C, A, B, alpha, beta, flags):
+    return lib().o_RankUpdate2Sym(_v(C), _v(A), _v(B), alpha, beta, flags)
+
+
+def UpdateTrm(C, A, B, alpha, beta, flags):
+    return lib().o_UpdateTrm(_v(C), _v(A), _v(B), alpha, beta, flags)
 
 
 def DecomposeCHOL(A, flags, nb):

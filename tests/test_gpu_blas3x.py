@@ -1,0 +1,257 @@
+"""GPU parity of the other BLAS-3 operations (MultSym, MultTrm,
+RankUpdate2Sym, UpdateTrm; mmult.go:266-520) through the C ABI, against the
+oracle on the same inputs (the reference's calgo kernels when built) and
+against numpy.
+
+Where the reference departs from the mathematics the GPU follows the
+mathematics and the comparison is with numpy, as its oracle pins record:
+MultSym RIGHT with a partial last 68-row block, MultTrm LEFT|UPPER|UNIT with
+alpha != 1, and beta != 1 in the triangular updates (the reference scales
+only its diagonal blocks). Bounds: 2 k eps |op(A)||op(B)| componentwise plus
+the beta term (both sides within k eps of the exact result).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import matops_amd as mo
+from matops_amd import _lib
+from helpers import EPS, embed, fmat
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ORACLE_KIND = "reference" if O.reference_available() else "port"
+
+
+def _md(X):
+    return _lib.MData(ctypes.c_void_p(X.ctypes.data), X.strides[1] // 8)
+
+
+def _sym_stored(n, upper, seed):
+    G = fmat(n, n, seed, "normal")
+    S = np.asfortranarray(G + G.T)
+    keep = np.triu(np.ones((n, n), bool)) if upper else np.tril(np.ones((n, n), bool))
+    return np.asfortranarray(np.where(keep, S, 3e250)), S
+
+
+def _oracle(fn, *args):
+    O.use(ORACLE_KIND)
+    try:
+        return fn(*args)
+    finally:
+        O.use("port")
+
+
+@pytest.mark.parametrize("side", [mo.LEFT, mo.RIGHT])
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+@pytest.mark.parametrize("n,m", [(1, 1), (90, 70), (136, 136), (700, 333)])
+def test_multsym(side, uplo, n, m):
+    if side == mo.RIGHT:
+        m = n  # the reference's size check admits RIGHT for square B only
+    A, S = _sym_stored(n, uplo == mo.UPPER, 11)
+    B = fmat(n, m, 12)
+    C0 = fmat(n, m, 13)
+    for alpha, beta in ((1.0, 0.0), (-1.5, 0.5)):
+        C = C0.copy(order="F")
+        if beta == 0.0:
+            C[:] = np.nan  # beta = 0 clears C
+        assert mo.MultSym(C, A, B, alpha, beta, side | uplo) is None
+        prod = S @ B if side == mo.LEFT else B @ S
+        absp = np.abs(S) @ np.abs(B) if side == mo.LEFT else np.abs(B) @ np.abs(S)
+        Cb = C0 if beta != 0.0 else np.zeros_like(C0)
+        bound = 2 * n * EPS * abs(alpha) * absp + 2 * EPS * np.abs(beta * Cb) + 1e-300
+        assert np.all(np.abs(C - (beta * Cb + alpha * prod)) <= bound)
+        if side == mo.LEFT or n % 68 == 0:
+            Co = Cb.copy(order="F")
+            _oracle(O.MultSym, Co, A, B, alpha, beta, side | uplo)
+            assert np.all(np.abs(C - Co) <= 2 * bound)
+
+
+def test_multsym_threads_and_views():
+    n, m = 520, 610
+    A, S = _sym_stored(n, False, 21)
+    B = fmat(n, m, 22)
+    Cp, Cv = embed(np.full((n, m), np.nan), 7, 3, 2)
+    old = mo.NumWorkers(4)
+    try:
+        assert mo.MultSym(Cv, A, B, 1.0, 0.0, mo.LEFT | mo.LOWER) is None
+    finally:
+        mo.NumWorkers(old)
+    assert np.all(np.abs(Cv - S @ B) <= 2 * n * EPS * (np.abs(S) @ np.abs(B)))
+    mask = np.ones(Cp.shape, bool)
+    mask[3:3 + n, 2:2 + m] = False
+    assert np.all(np.isnan(Cp[mask]))
+
+
+def _tri_operand(n, upper, unit, seed):
+    T = fmat(n, n, seed, "normal")
+    tri = np.triu(T) if upper else np.tril(T)
+    other = np.tril(np.full((n, n), 5e250), -1) if upper else np.triu(np.full((n, n), 5e250), 1)
+    stored = tri + other
+    eff = tri.copy()
+    if unit:
+        stored[np.diag_indices(n)] = np.nan
+        eff[np.diag_indices(n)] = 1.0
+    return np.asfortranarray(stored), eff
+
+
+@pytest.mark.parametrize("side", [mo.LEFT, mo.RIGHT])
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+@pytest.mark.parametrize("trans", [0, mo.TRANSA])
+@pytest.mark.parametrize("unit", [0, mo.UNIT])
+@pytest.mark.parametrize("n,m", [(80, 33), (700, 150)])
+def test_multtrm(side, uplo, trans, unit, n, m):
+    A, T = _tri_operand(n, uplo == mo.UPPER, unit, 31)
+    opT = T.T if trans else T
+    B0 = fmat(n, m, 32) if side == mo.LEFT else fmat(m, n, 32)
+    flags = side | uplo | trans | unit
+    B = B0.copy(order="F")
+    assert mo.MultTrm(B, A, -2.0, flags) is None
+    ref = -2.0 * (opT @ B0 if side == mo.LEFT else B0 @ opT)
+    absr = 2.0 * (np.abs(opT) @ np.abs(B0) if side == mo.LEFT else np.abs(B0) @ np.abs(opT))
+    assert np.all(np.abs(B - ref) <= 2 * n * EPS * absr + 1e-300)
+    defect = (ORACLE_KIND == "reference" and side == mo.LEFT and uplo == mo.UPPER and unit
+              and not trans)
+    if not defect:
+        Bo = B0.copy(order="F")
+        _oracle(O.MultTrm, Bo, A, -2.0, flags)
+        assert np.all(np.abs(B - Bo) <= 4 * n * EPS * absr + 1e-300)
+
+
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+@pytest.mark.parametrize("trans", [0, mo.TRANSA])
+@pytest.mark.parametrize("n,k", [(150, 40), (1000, 256)])
+def test_rankupdate2sym(uplo, trans, n, k):
+    A = fmat(*((k, n) if trans else (n, k)), seed=41)
+    B = fmat(*((k, n) if trans else (n, k)), seed=42)
+    C0 = fmat(n, n, seed=43)
+    oA, oB = (A.T, B.T) if trans else (A, B)
+    absr = np.abs(oA) @ np.abs(oB).T + np.abs(oB) @ np.abs(oA).T
+    tri = np.tril_indices(n) if uplo == mo.LOWER else np.triu_indices(n)
+    other = np.triu_indices(n, 1) if uplo == mo.LOWER else np.tril_indices(n, -1)
+    for beta in (1.0, 0.5):
+        C = C0.copy(order="F")
+        assert mo.RankUpdate2Sym(C, A, B, -1.0, beta, uplo | trans) is None
+        ref = beta * C0 - (oA @ oB.T + oB @ oA.T)
+        bound = 2 * k * EPS * absr + 2 * EPS * np.abs(C0)
+        assert np.all((np.abs(C - ref) <= bound)[tri])
+        np.testing.assert_array_equal(C[other], C0[other])
+        if beta == 1.0:
+            Co = C0.copy(order="F")
+            _oracle(O.RankUpdate2Sym, Co, A, B, -1.0, beta, uplo | trans)
+            assert np.all((np.abs(C - Co) <= 2 * bound)[tri])
+
+
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+@pytest.mark.parametrize("flags", [0, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
+@pytest.mark.parametrize("n,k", [(140, 50), (900, 300)])
+def test_updatetrm(uplo, flags, n, k):
+    ta, tb = flags & mo.TRANSA, flags & mo.TRANSB
+    A = fmat(*((k, n) if ta else (n, k)), seed=61)
+    B = fmat(*((n, k) if tb else (k, n)), seed=62)
+    C0 = fmat(n, n, seed=63)
+    oA = A.T if ta else A
+    oB = B.T if tb else B
+    absr = np.abs(oA) @ np.abs(oB)
+    tri = np.tril_indices(n) if uplo == mo.LOWER else np.triu_indices(n)
+    other = np.triu_indices(n, 1) if uplo == mo.LOWER else np.tril_indices(n, -1)
+    for beta in (1.0, -0.25):
+        C = C0.copy(order="F")
+        assert mo.UpdateTrm(C, A, B, -1.0, beta, uplo | flags) is None
+        ref = beta * C0 - oA @ oB
+        bound = 2 * k * EPS * absr + 2 * EPS * np.abs(C0)
+        assert np.all((np.abs(C - ref) <= bound)[tri])
+        np.testing.assert_array_equal(C[other], C0[other])
+        if beta == 1.0:
+            Co = C0.copy(order="F")
+            _oracle(O.UpdateTrm, Co, A, B, -1.0, beta, uplo | flags)
+            assert np.all((np.abs(C - Co) <= 2 * bound)[tri])
+
+
+def test_kernel_level_block_indices():
+    """The kernel-level entry points on sub-blocks: dmult_symm_blocked2 on
+    C[R:E, S:L], dmmat_trmm_blk on columns S:E, dmmat_rank2_blk and
+    dmmat_trmupd_blk on C[S:E, S:E] (UPPER, where the reference's indexing
+    is consistent), each against numpy; outside the block untouched."""
+    Lb = _lib.lib()
+    n, m, R, E, S, Lc = 200, 150, 11, 177, 7, 131
+    A, Sm = _sym_stored(n, True, 71)
+    B = fmat(n, m, 72)
+    C0 = fmat(n, m, 73)
+    C = C0.copy(order="F")
+    assert Lb.mops_dmult_symm_blocked2(ctypes.byref(_md(C)), ctypes.byref(_md(A)),
+                                       ctypes.byref(_md(B)), 2.0, -1.0, mo.LEFT | mo.UPPER, n, S,
+                                       Lc, R, E, 196, 68, 68) == 0
+    ref = C0.copy()
+    ref[R:E, S:Lc] = -C0[R:E, S:Lc] + 2.0 * Sm[R:E] @ B[:, S:Lc]
+    assert np.abs(C - ref).max() <= 4 * n * EPS * (np.abs(Sm) @ np.abs(B)).max()
+    mask = np.ones(C.shape, bool)
+    mask[R:E, S:Lc] = False
+    np.testing.assert_array_equal(C[mask], C0[mask])
+
+    At, T = _tri_operand(n, False, False, 74)
+    B0 = fmat(n, m, 75)
+    Bt = B0.copy(order="F")
+    assert Lb.mops_dmmat_trmm_blk(ctypes.byref(_md(Bt)), ctypes.byref(_md(At)), 1.0,
+                                  mo.LEFT | mo.LOWER, n, S, Lc, 68) == 0
+    ref = B0.copy()
+    ref[:, S:Lc] = T @ B0[:, S:Lc]
+    assert np.abs(Bt - ref).max() <= 4 * n * EPS * (np.abs(T) @ np.abs(B0)).max()
+
+    k, s0, e0 = 60, 40, 190
+    X, Y = fmat(n, k, 76), fmat(n, k, 77)
+    C0 = fmat(n, n, 78)
+    for fn, flags, prod in (("mops_dmmat_rank2_blk", mo.UPPER,
+                             lambda: X[s0:e0] @ Y[s0:e0].T + Y[s0:e0] @ X[s0:e0].T),
+                            ("mops_dmmat_trmupd_blk", mo.UPPER | mo.TRANSB,
+                             lambda: X[s0:e0] @ Y[s0:e0].T)):
+        C = C0.copy(order="F")
+        assert getattr(Lb, fn)(ctypes.byref(_md(C)), ctypes.byref(_md(X)), ctypes.byref(_md(Y)),
+                               1.0, 1.0, flags, k, s0, e0, 196, 68) == 0, fn
+        ref = C0.copy()
+        blk = C0[s0:e0, s0:e0] + prod()
+        iu = np.triu_indices(e0 - s0)
+        ref[s0:e0, s0:e0][iu] = blk[iu]
+        np.testing.assert_allclose(C, ref, rtol=0, atol=4 * k * k * EPS, err_msg=fn)
+
+
+def test_device_twins_match_host_calls():
+    import torch
+    dev = torch.device("cuda:0")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    Lb = _lib.lib()
+
+    def d(X):
+        return torch.from_numpy(np.ascontiguousarray(X.T)).to(dev).T
+
+    n, m = 300, 170
+    A, Sm = _sym_stored(n, False, 81)
+    B, C0 = fmat(n, m, 82), fmat(n, m, 83)
+    C = C0.copy(order="F")
+    mo.MultSym(C, A, B, 1.0, 0.5, mo.LEFT | mo.LOWER)
+    dA, dB, dC = d(A), d(B), d(C0)
+    assert Lb.mops_d_dsymm(mo.LEFT | mo.LOWER, n, m, 1.0, dA.data_ptr(), n, dB.data_ptr(), n, 0.5,
+                           dC.data_ptr(), n, s) == 0
+    np.testing.assert_array_equal(dC.cpu().numpy(), C)
+
+    At, _ = _tri_operand(n, True, True, 84)
+    Bt = B.copy(order="F")
+    mo.MultTrm(Bt, At, 1.5, mo.LEFT | mo.UPPER | mo.UNIT | mo.TRANSA)
+    dAt, dBt = d(At), d(B)
+    assert Lb.mops_d_dtrmm(mo.LEFT | mo.UPPER | mo.UNIT | mo.TRANSA, n, m, 1.5, dAt.data_ptr(), n,
+                           dBt.data_ptr(), n, s) == 0
+    np.testing.assert_array_equal(dBt.cpu().numpy(), Bt)
+
+    k = 90
+    X, Y, Cs = fmat(n, k, 85), fmat(n, k, 86), fmat(n, n, 87)
+    for name, fn, flags in (("mops_d_dsyr2k", mo.RankUpdate2Sym, mo.LOWER),
+                            ("mops_d_dtrmupd", mo.UpdateTrm, mo.LOWER | mo.TRANSB)):
+        C = Cs.copy(order="F")
+        fn(C, X, Y, -1.0, 1.0, flags)
+        dX, dY, dC = d(X), d(Y), d(Cs)
+        assert getattr(Lb, name)(flags, n, k, -1.0, dX.data_ptr(), n, dY.data_ptr(), n, 1.0,
+                                 dC.data_ptr(), n, s) == 0
+        np.testing.assert_array_equal(dC.cpu().numpy(), C, err_msg=name)
+    torch.cuda.synchronize()
+    assert Lb.mops_d_status(s) == 0
