@@ -283,6 +283,39 @@ int mops_d_dgetrf(int M, int N, int nb, double *A, int lda, int64_t *ipiv, int *
 int mops_d_dgetrs(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,
                   int npiv, const int *perm, double *B, int ldb, void *stream);
 
+/* ---- factorizations without pivoting, the triangular inverse ------------- */
+
+/* Replaces DecomposeLUnoPiv / blockedLUnoPiv / unblockedLUnoPiv (lu.go:38-96,
+ * 291-299): A (M x N) = L U in place, unit L below the diagonal, no row
+ * interchanges; a zero pivot gives Inf/NaN, not an error, as in the
+ * reference. nb is accepted and ignored (every blocking gives the same
+ * factors to rounding). Host memory, synchronous. */
+int mops_dgetrf_nopiv(mops_mdata_t *A, int M, int N, int nb);
+int mops_d_dgetrf_nopiv(int M, int N, int nb, double *A, int lda, void *stream);
+
+/* Replaces DecomposeLDLnoPiv and its blocked / unblocked forms
+ * (ldlnp.go:23-241): A (N x N) = L D L^T (LOWER) or U D U^T (otherwise) in
+ * place, unit L / U in the selected strict triangle, D on the diagonal, the
+ * other triangle not written. No workspace argument: the GPU keeps its own
+ * (the Go layer's "No workspace for blocking invocation" check stays in the
+ * caller). Host memory, synchronous. */
+int mops_dsytrf_nopiv(mops_mdata_t *A, int flags, int N, int nb);
+int mops_d_dsytrf_nopiv(int flags, int N, int nb, double *A, int lda, void *stream);
+
+/* Replaces SolveLDLnoPiv (ldlnp.go:251-264): B (N x nrhs) = A^-1 B from the
+ * factors of mops_dsytrf_nopiv; UPPER selects U D U^T, else LOWER L D L^T;
+ * neither bit: B is left as is. Host memory, synchronous. */
+int mops_dsytrs_nopiv(mops_mdata_t *B, const mops_mdata_t *A, int flags, int N, int nrhs);
+int mops_d_dsytrs_nopiv(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
+                        void *stream);
+
+/* Replaces InverseTrm (trinv.go:16-245): the LOWER (else upper) triangle of
+ * A (N x N) replaced by that of its inverse; UNIT takes the diagonal as 1
+ * and leaves it as stored; the other triangle is not written. nb is
+ * accepted and ignored. Host memory, synchronous. */
+int mops_dinverse_trm(mops_mdata_t *A, int flags, int N, int nb);
+int mops_d_dinverse_trm(int flags, int N, int nb, double *A, int lda, void *stream);
+
 /* ---- Mixed precision LU with iterative refinement ------------------------ */
 
 /* No counterpart in the reference: for callers that want the LU solve of a

@@ -9,4 +9,5 @@ from .matops import (  # noqa: F401
     BlockingParams, CholeskyAdjoint, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
     PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
     blockIndex4, divideWork, scheduleWork, MultSym, MultTrm, RankUpdate2Sym, UpdateTrm,
+    DecomposeLUnoPiv, DecomposeLDLnoPiv, SolveLDLnoPiv, InverseTrm,
 )

@@ -54,6 +54,14 @@ SIGNATURES = {
     "mops_d_dsyr2k": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
     "mops_dmmat_trmupd_blk": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I]),
     "mops_d_dtrmupd": (_I, [_I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
+    "mops_dgetrf_nopiv": (_I, [_PM, _I, _I, _I]),
+    "mops_d_dgetrf_nopiv": (_I, [_I, _I, _I, _P, _I, _P]),
+    "mops_dsytrf_nopiv": (_I, [_PM, _I, _I, _I]),
+    "mops_d_dsytrf_nopiv": (_I, [_I, _I, _I, _P, _I, _P]),
+    "mops_dsytrs_nopiv": (_I, [_PM, _PM, _I, _I, _I]),
+    "mops_d_dsytrs_nopiv": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+    "mops_dinverse_trm": (_I, [_PM, _I, _I, _I]),
+    "mops_d_dinverse_trm": (_I, [_I, _I, _I, _P, _I, _P]),
     "mops_dpotrf": (_I, [_PM, _I, _I, _I]),
     "mops_dpotrs": (_I, [_PM, _PM, _I, _I, _I]),
     "mops_d_dpotrf": (_I, [_I, _I, _I, _P, _I, _P, _P]),

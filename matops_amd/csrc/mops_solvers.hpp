@@ -62,6 +62,23 @@ int syr2k_d(hipStream_t s, int flags, int N, int P, double alpha, const double* 
 int trmupd_d(hipStream_t s, int flags, int N, int P, double alpha, const double* A, long lda,
              const double* B, long ldb, double beta, double* C, long ldc);
 
+// Factorizations without pivoting and the triangular inverse (factor_x.hip).
+// getrf_nopiv_d: A (m x n) = L U, unit L below the diagonal, no row
+// interchanges. sytrf_nopiv_d: A (n x n) = L D L^T (lower) or U D U^T, unit
+// L / U off the diagonal, D on it, the other triangle untouched;
+// sytrs_nopiv_d: B = A^-1 B from those factors. inverse_trm_d: the lower
+// (else upper) triangle of A replaced by that of its inverse; with unit the
+// diagonal is taken as 1 and left as stored.
+size_t getrf_nopiv_ws_bytes(int m, int n);
+int getrf_nopiv_d(hipStream_t s, int m, int n, double* A, long lda);
+size_t sytrf_nopiv_ws_bytes(int n);
+int sytrf_nopiv_d(hipStream_t s, bool lower, int n, double* A, long lda);
+size_t sytrs_nopiv_ws_bytes(int n, int nrhs);
+int sytrs_nopiv_d(hipStream_t s, bool lower, int n, int nrhs, const double* A, long lda,
+                  double* B, long ldb);
+size_t inverse_trm_ws_bytes(int n);
+int inverse_trm_d(hipStream_t s, bool lower, bool unit, int n, double* A, long lda);
+
 // Cholesky of the N x N matrix A in place (LOWER or UPPER flag), outer block
 // nb. `info` (device int, may be null) receives the 0-based global index of
 // the first NaN square root, or stays -1. `badval` (device double, may be

@@ -395,6 +395,55 @@ def SolveIR(X, A, B, prec=PREC_FP32, tol=1e-12, maxit=10):
     return {f: getattr(rep, f) for f, _ in rep._fields_}
 
 
+def DecomposeLUnoPiv(A, nb):
+    """LU without pivoting in place (lu.go:291-299). Returns (A, err)."""
+    if min(A.shape) == 0:
+        return A, None
+    Am, _, _ = _view(A)
+    check("mops_dgetrf_nopiv", _lib.lib().mops_dgetrf_nopiv(
+        ctypes.byref(Am), A.shape[0], A.shape[1], nb))
+    return A, None
+
+
+def DecomposeLDLnoPiv(A, W, flags, nb):
+    """A = L D L^T (LOWER) or U D U^T without pivoting, in place
+    (ldlnp.go:211-241). W: the workspace the reference's blocked form needs
+    (checked as the reference does; the GPU keeps its own). Returns (A, err)
+    with A None on error, as the reference."""
+    if A.shape[0] != A.shape[1]:
+        return None, MatopsError("A not a square matrix")
+    n = A.shape[0]
+    if not (n < nb or nb == 0) and W is None:
+        return None, MatopsError("No workspace for blocking invocation")
+    if n == 0:
+        return A, None
+    Am, _, _ = _view(A)
+    check("mops_dsytrf_nopiv", _lib.lib().mops_dsytrf_nopiv(ctypes.byref(Am), flags, n, nb))
+    return A, None
+
+
+def SolveLDLnoPiv(B, A, flags):
+    """B = A^-1 B from DecomposeLDLnoPiv's factors (ldlnp.go:251-264)."""
+    n = A.shape[0]
+    if n == 0 or B.shape[1] == 0:
+        return
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dsytrs_nopiv", _lib.lib().mops_dsytrs_nopiv(
+        ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
+
+
+def InverseTrm(A, flags, nb):
+    """Inverse of the LOWER (else upper) triangle of A in place, UNIT for a
+    unit diagonal (trinv.go:219-245). Returns (A, err)."""
+    n = A.shape[0]
+    if n == 0:
+        return A, None
+    Am, _, _ = _view(A)
+    check("mops_dinverse_trm", _lib.lib().mops_dinverse_trm(ctypes.byref(Am), flags, n, nb))
+    return A, None
+
+
 def DecomposeLU(A, pivots, nb):
     """LU with partial pivoting in place (lu.go:257-273). pivots: int64
     numpy array (Go []int), at least min(rows, cols) long. Returns (A, err)."""

@@ -588,6 +588,270 @@ int o_SolveLU(oview B, oview A, const int64_t *pivots, int npiv, int flags) {
   return 0;
 }
 
+/* ---- rank-1 helpers of the no-pivot factorizations and the inverse ----
+ * MVRankUpdate (mvec.go:67-99 -> dmvec_rank) and MVUpdateTrm (mvec.go:163-
+ * 192 -> dmvec_trmv_upd) restated as plain loops: A += alpha x y^T on the
+ * whole block, or on its LOWER / UPPER triangle. Scale multiplies. */
+static double velem(oview X, int k) { /* element k of a row or column vector view */
+  return X.rows == 1 ? X.p[(ptrdiff_t)k * X.ld] : X.p[k];
+}
+
+static void MVRankUpdate(oview A, oview X, oview Y, double alpha) {
+  for (int j = 0; j < A.cols; j++) {
+    const double yj = alpha * velem(Y, j);
+    for (int i = 0; i < A.rows; i++) A.p[i + (ptrdiff_t)j * A.ld] += velem(X, i) * yj;
+  }
+}
+
+static void MVUpdateTrm(oview A, oview X, oview Y, double alpha, int flags) {
+  for (int j = 0; j < A.cols; j++) {
+    const double yj = alpha * velem(Y, j);
+    const int i0 = flags & O_UPPER ? 0 : j, i1 = flags & O_UPPER ? j + 1 : A.rows;
+    for (int i = i0; i < i1 && i < A.rows; i++) A.p[i + (ptrdiff_t)j * A.ld] += velem(X, i) * yj;
+  }
+}
+
+static void Scale(oview X, double alpha) { /* matrix.FloatMatrix.Scale on a vector view */
+  for (int k = 0; k < numel(X); k++) {
+    double *p = X.rows == 1 ? &X.p[(ptrdiff_t)k * X.ld] : &X.p[k];
+    *p *= alpha;
+  }
+}
+
+/* ---- DecomposeLUnoPiv, lu.go:38-96, 291-299 ---- */
+static void unblockedLUnoPiv(oview A) {
+  /* FLAME LU variant 5 (lu.go:38-61): a21 /= a11; A22 -= a21 a12 */
+  for (int k = 0; k < A.rows && k < A.cols; k++) {
+    oview a11 = sub(A, k, k, 1, 1);
+    oview a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    oview a12 = sub(A, k, k + 1, 1, A.cols - k - 1);
+    InvScale(a21, a11.p[0]);
+    MVRankUpdate(sub(A, k + 1, k + 1, A.rows - k - 1, A.cols - k - 1), a21, a12, -1.0);
+  }
+}
+
+static void blockedLUnoPiv(oview A, int nb) {
+  /* lu.go:64-96 */
+  for (int k = 0; k < A.rows && k < A.cols;) {
+    int kb = k + nb > A.rows ? A.rows - k : nb; /* partition.go:252-254 (rows and cols) */
+    if (k + kb > A.cols) kb = A.cols - k;
+    oview A11 = sub(A, k, k, kb, kb);
+    oview A12 = sub(A, k, k + kb, kb, A.cols - k - kb);
+    oview A21 = sub(A, k + kb, k, A.rows - k - kb, kb);
+    oview A22 = sub(A, k + kb, k + kb, A.rows - k - kb, A.cols - k - kb);
+    unblockedLUnoPiv(A11);
+    o_SolveTrm(A12, A11, 1.0, O_LEFT | O_LOWER | O_UNIT);
+    o_SolveTrm(A21, A11, 1.0, O_RIGHT | O_UPPER);
+    o_Mult(A22, A21, A12, -1.0, 1.0, 0);
+    k += kb;
+  }
+}
+
+int o_DecomposeLUnoPiv(oview A, int nb) {
+  int mlen = A.rows < A.cols ? A.rows : A.cols;
+  if (mlen <= nb || nb == 0)
+    unblockedLUnoPiv(A);
+  else
+    blockedLUnoPiv(A, nb);
+  return 0;
+}
+
+/* ---- DecomposeLDLnoPiv / SolveLDLnoPiv, ldlnp.go:23-264 ---- */
+static void unblkLowerLDLnoPiv(oview A) {
+  /* ldlnp.go:23-61: A22 -= a21 a21^T / a11 (lower), then l21 = a21 / a11 */
+  for (int k = 0; k < A.rows; k++) {
+    oview a11 = sub(A, k, k, 1, 1);
+    oview a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    MVUpdateTrm(sub(A, k + 1, k + 1, A.rows - k - 1, A.cols - k - 1), a21, a21, -1.0 / a11.p[0],
+                O_LOWER);
+    InvScale(a21, a11.p[0]);
+  }
+}
+
+static void unblkUpperLDLnoPiv(oview A) {
+  /* ldlnp.go:64-101: from the last column: A00 -= a01 a01^T / a11 (upper),
+   * then u01 = a01 / a11 */
+  for (int k = A.rows - 1; k >= 0; k--) {
+    oview a11 = sub(A, k, k, 1, 1);
+    oview a01 = sub(A, 0, k, k, 1);
+    MVUpdateTrm(sub(A, 0, 0, k, k), a01, a01, -1.0 / a11.p[0], O_UPPER);
+    InvScale(a01, a11.p[0]);
+  }
+}
+
+/* SolveDiag(X, D, RIGHT) with D the diagonal of A11: column k times 1/d_k
+ * (diag.go:96-101, 117-122) */
+static void solveDiagRight(oview X, oview A11) {
+  for (int k = 0; k < X.cols; k++) Scale(sub(X, 0, k, X.rows, 1), 1.0 / A11.p[(ptrdiff_t)k * (A11.ld + 1)]);
+}
+
+static void blkLowerLDLnoPiv(oview A, int nb) {
+  /* ldlnp.go:104-145 */
+  for (int k = 0; k < A.rows;) {
+    int kb = k + nb > A.rows ? A.rows - k : nb;
+    oview A11 = sub(A, k, k, kb, kb);
+    oview A21 = sub(A, k + kb, k, A.rows - k - kb, kb);
+    oview A22 = sub(A, k + kb, k + kb, A.rows - k - kb, A.rows - k - kb);
+    unblkLowerLDLnoPiv(A11);
+    o_SolveTrm(A21, A11, 1.0, O_LOWER | O_UNIT | O_RIGHT | O_TRANSA);
+    solveDiagRight(A21, A11);
+    /* wrk = A21 D1 (MultDiag RIGHT, diag.go) */
+    double *w = malloc(sizeof(double) * (size_t)(A21.rows > 0 ? A21.rows : 1) * kb);
+    oview W = {w, A21.rows, kb, A21.rows > 0 ? A21.rows : 1};
+    for (int j = 0; j < kb; j++)
+      for (int i = 0; i < A21.rows; i++)
+        w[i + (ptrdiff_t)j * W.ld] = A21.p[i + (ptrdiff_t)j * A21.ld] * A11.p[(ptrdiff_t)j * (A11.ld + 1)];
+    o_UpdateTrm(A22, A21, W, -1.0, 1.0, O_LOWER | O_TRANSB);
+    free(w);
+    k += kb;
+  }
+}
+
+static void blkUpperLDLnoPiv(oview A, int nb) {
+  /* ldlnp.go:148-208: blocks from the bottom right */
+  for (int e = A.rows; e > 0;) {
+    int kb = e - nb < 0 ? e : nb;
+    int k = e - kb;
+    oview A00 = sub(A, 0, 0, k, k);
+    oview A01 = sub(A, 0, k, k, kb);
+    oview A11 = sub(A, k, k, kb, kb);
+    unblkUpperLDLnoPiv(A11);
+    o_SolveTrm(A01, A11, 1.0, O_UPPER | O_UNIT | O_RIGHT | O_TRANSA);
+    solveDiagRight(A01, A11);
+    double *w = malloc(sizeof(double) * (size_t)(k > 0 ? k : 1) * kb);
+    oview W = {w, k, kb, k > 0 ? k : 1};
+    for (int j = 0; j < kb; j++)
+      for (int i = 0; i < k; i++)
+        w[i + (ptrdiff_t)j * W.ld] = A01.p[i + (ptrdiff_t)j * A01.ld] * A11.p[(ptrdiff_t)j * (A11.ld + 1)];
+    o_UpdateTrm(A00, A01, W, -1.0, 1.0, O_UPPER | O_TRANSB);
+    free(w);
+    e = k;
+  }
+}
+
+int o_DecomposeLDLnoPiv(oview A, int flags, int nb) {
+  /* ldlnp.go:211-241 (the workspace W of the blocked form is allocated
+   * here; a missing W is the Go layer's error) */
+  if (A.cols != A.rows) return 1; /* "A not a square matrix" */
+  if (A.cols < nb || nb == 0) {
+    if (flags & O_LOWER)
+      unblkLowerLDLnoPiv(A);
+    else
+      unblkUpperLDLnoPiv(A);
+  } else {
+    if (flags & O_LOWER)
+      blkLowerLDLnoPiv(A, nb);
+    else
+      blkUpperLDLnoPiv(A, nb);
+  }
+  return 0;
+}
+
+void o_SolveLDLnoPiv(oview B, oview A, int flags) {
+  /* ldlnp.go:251-264; SolveDiag(B, A, LEFT) with the matrix A: row k times
+   * 1/a_kk (diag.go:111-116) */
+  int tri = flags & O_UPPER ? O_UPPER : (flags & O_LOWER ? O_LOWER : 0);
+  if (!tri) return;
+  o_SolveTrm(B, A, 1.0, tri | O_UNIT);
+  for (int k = 0; k < B.rows; k++) Scale(sub(B, k, 0, 1, B.cols), 1.0 / A.p[(ptrdiff_t)k * (A.ld + 1)]);
+  o_SolveTrm(B, A, 1.0, tri | O_UNIT | O_TRANSA);
+}
+
+/* ---- InverseTrm, trinv.go:16-245 ---- */
+static void unblockedInverseLower(oview A) { /* trinv.go:16-48 */
+  for (int k = 0; k < A.rows; k++) {
+    oview a10 = sub(A, k, 0, 1, k), a11 = sub(A, k, k, 1, 1);
+    oview A20 = sub(A, k + 1, 0, A.rows - k - 1, k), a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    double aval = a11.p[0];
+    InvScale(a21, -aval);
+    MVRankUpdate(A20, a21, a10, 1.0);
+    InvScale(a10, aval);
+    a11.p[0] = 1.0 / aval;
+  }
+}
+
+static void unblockedInverseUpper(oview A) { /* trinv.go:51-83 */
+  for (int k = 0; k < A.rows; k++) {
+    oview a01 = sub(A, 0, k, k, 1), a11 = sub(A, k, k, 1, 1);
+    oview a12 = sub(A, k, k + 1, 1, A.cols - k - 1), A02 = sub(A, 0, k + 1, k, A.cols - k - 1);
+    double aval = a11.p[0];
+    InvScale(a12, -aval);
+    MVRankUpdate(A02, a01, a12, 1.0);
+    InvScale(a01, aval);
+    a11.p[0] = 1.0 / aval;
+  }
+}
+
+static void unblockedInverseUnitLower(oview A) { /* trinv.go:86-113 */
+  for (int k = 0; k < A.rows; k++) {
+    oview a10 = sub(A, k, 0, 1, k);
+    oview A20 = sub(A, k + 1, 0, A.rows - k - 1, k), a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    Scale(a21, -1.0);
+    MVRankUpdate(A20, a21, a10, 1.0);
+  }
+}
+
+static void unblockedInverseUnitUpper(oview A) { /* trinv.go:116-143 */
+  for (int k = 0; k < A.rows; k++) {
+    oview a01 = sub(A, 0, k, k, 1);
+    oview a12 = sub(A, k, k + 1, 1, A.cols - k - 1), A02 = sub(A, 0, k + 1, k, A.cols - k - 1);
+    Scale(a12, -1.0);
+    MVRankUpdate(A02, a01, a12, 1.0);
+  }
+}
+
+static void blockedInverseLower(oview A, int flags, int nb) { /* trinv.go:146-180 */
+  for (int k = 0; k < A.rows;) {
+    int kb = k + nb > A.rows ? A.rows - k : nb;
+    oview A10 = sub(A, k, 0, kb, k), A11 = sub(A, k, k, kb, kb);
+    oview A20 = sub(A, k + kb, 0, A.rows - k - kb, k), A21 = sub(A, k + kb, k, A.rows - k - kb, kb);
+    o_SolveTrm(A21, A11, -1.0, flags | O_RIGHT);
+    o_Mult(A20, A21, A10, 1.0, 1.0, 0);
+    o_SolveTrm(A10, A11, 1.0, flags);
+    if (flags & O_UNIT)
+      unblockedInverseUnitLower(A11);
+    else
+      unblockedInverseLower(A11);
+    k += kb;
+  }
+}
+
+static void blockedInverseUpper(oview A, int flags, int nb) { /* trinv.go:183-216 */
+  for (int k = 0; k < A.rows;) {
+    int kb = k + nb > A.rows ? A.rows - k : nb;
+    oview A00 = sub(A, 0, 0, k, k), A01 = sub(A, 0, k, k, kb), A11 = sub(A, k, k, kb, kb);
+    o_MultTrm(A01, A00, 1.0, flags);
+    o_SolveTrm(A01, A11, -1.0, flags | O_RIGHT);
+    if (flags & O_UNIT)
+      unblockedInverseUnitUpper(A11);
+    else
+      unblockedInverseUpper(A11);
+    k += kb;
+  }
+}
+
+int o_InverseTrm(oview A, int flags, int nb) { /* trinv.go:219-245 */
+  if (nb == 0 || A.cols < nb) {
+    if (flags & O_UNIT) {
+      if (flags & O_LOWER)
+        unblockedInverseUnitLower(A);
+      else
+        unblockedInverseUnitUpper(A);
+    } else {
+      if (flags & O_LOWER)
+        unblockedInverseLower(A);
+      else
+        unblockedInverseUpper(A);
+    }
+  } else {
+    if (flags & O_LOWER)
+      blockedInverseLower(A, flags, nb);
+    else
+      blockedInverseUpper(A, flags, nb);
+  }
+  return 0;
+}
+
 /* ---- reference Cholesky adjoint (no counterpart in matops) ----
  * Abar = 1/2 (S + S^T), S = L^-T Phi(L^T Lbar) L^-1 (the closed form of the
  * Cholesky derivative, e.g. Murray 2016, arXiv:1602.07527, eq. 10), computed

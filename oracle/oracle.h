@@ -86,6 +86,10 @@ int o_MultSym(oview C, oview A, oview B, double alpha, double beta, int flags);
 int o_MultTrm(oview B, oview A, double alpha, int flags);
 int o_RankUpdate2Sym(oview C, oview A, oview B, double alpha, double beta, int flags);
 int o_UpdateTrm(oview C, oview A, oview B, double alpha, double beta, int flags);
+int o_DecomposeLUnoPiv(oview A, int nb);
+int o_DecomposeLDLnoPiv(oview A, int flags, int nb);
+void o_SolveLDLnoPiv(oview B, oview A, int flags);
+int o_InverseTrm(oview A, int flags, int nb);
 
 /* Reference adjoint of the Cholesky factorization (closed form, built on
  * the oracle's TRSM/GEMM): given the lower factor L and Lbar (lower),

@@ -63,6 +63,10 @@ def lib():
             "o_MultTrm": (I, [V, V, D, I]),
             "o_RankUpdate2Sym": (I, [V, V, V, D, D, I]),
             "o_UpdateTrm": (I, [V, V, V, D, D, I]),
+            "o_DecomposeLUnoPiv": (I, [V, I]),
+            "o_DecomposeLDLnoPiv": (I, [V, I, I]),
+            "o_SolveLDLnoPiv": (None, [V, V, I]),
+            "o_InverseTrm": (I, [V, I, I]),
             "o_chol_adjoint": (None, [I, P, I, P, I, P, I]),
             "o_now": (D, []),
             "o_kern_gemm": (None, [P, I, P, I, P, I, D, D, I, I, I, I, I, I]),
@@ -141,6 +145,22 @@ def RankUpdate2Sym(C, A, B, alpha, beta, flags):
 
 def UpdateTrm(C, A, B, alpha, beta, flags):
     return lib().o_UpdateTrm(_v(C), _v(A), _v(B), alpha, beta, flags)
+
+
+def DecomposeLUnoPiv(A, nb):
+    return lib().o_DecomposeLUnoPiv(_v(A), nb)
+
+
+def DecomposeLDLnoPiv(A, flags, nb):
+    return lib().o_DecomposeLDLnoPiv(_v(A), flags, nb)
+
+
+def SolveLDLnoPiv(B, A, flags):
+    lib().o_SolveLDLnoPiv(_v(B), _v(A), flags)
+
+
+def InverseTrm(A, flags, nb):
+    return lib().o_InverseTrm(_v(A), flags, nb)
 
 
 def DecomposeCHOL(A, flags, nb):
