@@ -105,6 +105,17 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
   // lanes 0-31 of one ds_read_b64 fall in disjoint bank halves.
   constexpr int PAD = 16;
   constexpr int LDS_A = BM + PAD, LDS_B = BN + PAD;
+  // Column swizzle of the staged slabs: element (k, i) sits at column
+  // i ^ (4 * ((k >> 1) & 7)) of row k. An operand that is contiguous along k
+  // is staged as (k, k+1) pairs: eight neighbouring lanes write one column
+  // at rows 0, 2, ..., 14, which the pitch alone maps to one bank (an 8-way
+  // conflict on every slab store); the swizzle gives them 8 banks. It only
+  // flips bits 2-4 of the column, so element pairs stay contiguous, a row
+  // keeps its footprint, and the MFMA reads (16 consecutive columns of rows
+  // 4r..4r+3; rows 4r and 4r+1 share the swizzle) keep their bank halves.
+  auto swz = [](int kk, int i) { return i ^ (4 * ((kk >> 1) & 7)); };
+  auto posA = [&](int kk, int i) { return kk * LDS_A + swz(kk, i); };
+  auto posB = [&](int kk, int i) { return kk * LDS_B + swz(kk, i); };
   // operands move in pairs of consecutive elements along their contiguous
   // dimension (m or n when not transposed for A / transposed for B, else k)
   constexpr int A_PP = BM * BK / 2 / NT, B_PP = BN * BK / 2 / NT;
@@ -249,10 +260,10 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
       int i, kk;
       coords(e, BM, !TA, i, kk);
       if (!TA) {
-        *(pair_t*)&As[buf][kk * LDS_A + i] = xa[e];
+        *(pair_t*)&As[buf][posA(kk, i)] = xa[e];
       } else {
-        As[buf][kk * LDS_A + i] = xa[e].x;
-        As[buf][(kk + 1) * LDS_A + i] = xa[e].y;
+        As[buf][posA(kk, i)] = xa[e].x;
+        As[buf][posA(kk + 1, i)] = xa[e].y;
       }
     }
 #pragma unroll
@@ -260,10 +271,10 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
       int j, kk;
       coords(e, BN, TB, j, kk);
       if (TB) {
-        *(pair_t*)&Bs[buf][kk * LDS_B + j] = xb[e];
+        *(pair_t*)&Bs[buf][posB(kk, j)] = xb[e];
       } else {
-        Bs[buf][kk * LDS_B + j] = xb[e].x;
-        Bs[buf][(kk + 1) * LDS_B + j] = xb[e].y;
+        Bs[buf][posB(kk, j)] = xb[e].x;
+        Bs[buf][posB(kk + 1, j)] = xb[e].y;
       }
     }
   };
@@ -277,9 +288,9 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     for (int kk = kh * (BK / KS); kk < (kh + 1) * (BK / KS); kk += 4) {
       T af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = as[(kk + lk) * LDS_A + wm * WM + i * 16 + lr];
+      for (int i = 0; i < FM; ++i) af[i] = as[posA(kk + lk, wm * WM + i * 16 + lr)];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + lk) * LDS_B + wn * WN + j * 16 + lr];
+      for (int j = 0; j < FN; ++j) bf[j] = bs[posB(kk + lk, wn * WN + j * 16 + lr)];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
