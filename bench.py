@@ -105,8 +105,9 @@ def main():
         elif args.only == "lu":
             res = extra_lu(L, ok, colmajor, torch, dev, sp)[0]
         elif args.only == "mixed":
-            _, xref = extra_lu(L, ok, colmajor, torch, dev, sp, reps=0)
-            res = extra_mixed(L, ok, colmajor, torch, dev, sp, xref, reps=0)
+            lu, _ = extra_lu(L, ok, colmajor, torch, dev, sp, reps=0)
+            res = extra_mixed(L, ok, colmajor, torch, dev, sp, lapack_solution(torch, dev, colmajor),
+                              lu["total_ms"], reps=0)
         else:
             res = extra_adjoint(L, ok, colmajor, torch, dev, sp)
         print(json.dumps({args.only: res}), flush=True)
@@ -235,12 +236,15 @@ def main():
         del As, Bs
         torch.cuda.empty_cache()
         out["dgemm_1024"] = extra_dgemm(L, ok, colmajor, gen, stream, sp)
-        out["lu_16384"], xref = extra_lu(L, ok, colmajor, torch, dev, sp)
-        out["mixed_lu_16384"] = extra_mixed(L, ok, colmajor, torch, dev, sp, xref)
+        out["lu_16384"], xlu = extra_lu(L, ok, colmajor, torch, dev, sp)
+        xref = lapack_solution(torch, dev, colmajor)
+        out["lu_16384"]["forward_error_vs_lapack"] = _fwd(xlu, xref)
+        out["mixed_lu_16384"] = extra_mixed(L, ok, colmajor, torch, dev, sp, xref,
+                                            out["lu_16384"]["total_ms"])
         out["chol_adjoint_4096"] = extra_adjoint(L, ok, colmajor, torch, dev, sp)
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(nb)
+        out["cpu_baseline"] = cpu_baseline(nb, n, nrhs)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -350,15 +354,30 @@ def extra_lu(L, ok, colmajor, torch, dev, sp, n=16384, nrhs=16, reps=3):
     return (res, X)
 
 
-def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, n=16384, nrhs=16, reps=2):
-    """The same system by FP32 LU + FP64 refinement to eta <= 1e-12, next to
-    the plain FP64 solve (mops_d_dgesv_ir with precision FP64 and FP32); the
-    forward error is against the FP64 GPU solution (pivots identical to
-    LAPACK's, backward error O(eps))."""
+def _fwd(X, xref):
+    return ((X - xref).abs().amax() / xref.abs().amax()).item()
+
+
+def lapack_solution(torch, dev, colmajor, n=16384, nrhs=16):
+    """x = A^-1 B of the LU workload by LAPACK (scipy dgesv, FP64, on the
+    host cores), the reference solution for the forward errors."""
+    import scipy.linalg as sla
+    A0, B0 = _lu_problem(torch, dev, colmajor, n, nrhs)
+    A = A0.cpu().numpy()
+    B = B0.cpu().numpy()
+    x = sla.solve(A, B, overwrite_a=True, check_finite=False)
+    return torch.from_numpy(x).to(dev)
+
+
+def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, lu_total_ms, n=16384, nrhs=16, reps=2):
+    """The same system by LU in FP32 (and in BF16) + FP64 refinement to
+    eta <= 1e-12 (mops_d_dgesv_ir), next to the same entry point in FP64;
+    the speed-up is taken against the plain FP64 DecomposeLU + SolveLU wall
+    time (lu_total_ms). Forward errors are against LAPACK's FP64 solution."""
     from matops_amd import _lib
     A0, B0 = _lu_problem(torch, dev, colmajor, n, nrhs)
     res = {}
-    for name, prec in (("fp64", 0), ("fp32_ir", 1)):
+    for name, prec in (("fp64", 0), ("fp32_ir", 1), ("bf16_ir", 2)):
         best = None
         for _ in range(reps + 1):
             X = colmajor(n, nrhs)
@@ -372,13 +391,13 @@ def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, n=16384, nrhs=16, reps=2)
             if best is None or t < best[0]:
                 best = (t, rep.iterations, rep.backward_error, rep.converged, rep.factor_ms,
                         rep.refine_ms)
-        fwd = ((X - xref).abs().amax() / xref.abs().amax()).item()
         res[name] = {"wall_ms": round(best[0] * 1e3, 2), "iterations": best[1],
                      "backward_error": best[2], "converged": bool(best[3]),
                      "factor_ms": round(best[4], 2), "refine_ms": round(best[5], 2),
-                     "forward_error_vs_fp64": fwd,
+                     "forward_error_vs_lapack": _fwd(X, xref),
                      "backward_error_check": _eta(torch, A0, X, B0)}
-    res["speedup_vs_fp64"] = round(res["fp64"]["wall_ms"] / res["fp32_ir"]["wall_ms"], 3)
+    res["plain_fp64_lu_total_ms"] = lu_total_ms
+    res["speedup_fp32_ir_vs_plain_fp64_lu"] = round(lu_total_ms / res["fp32_ir"]["wall_ms"], 3)
     return res
 
 
@@ -464,13 +483,21 @@ def traffic_summary(fetch_db, write_db):
     return 0
 
 
-def cpu_baseline(nb):
-    """The CPU oracle on a bounded sample of the same workload."""
+def cpu_baseline(nb, n, nrhs):
+    """The CPU oracle (the reference's calgo kernels under the restated Go
+    glue) on the host: the Cholesky workload itself on one core (value), and
+    the DGEMM 1024^3 and a bounded LU sample (n = 4096, 16 RHS) with one and
+    with all host cores as Mult workers."""
     try:
         from oracle import oracle as orc
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "TFLOP/s", "error": f"oracle unavailable: {e}"}
-    return orc.bench_cholesky_sample(nb=nb)
+    out = orc.bench_cholesky_sample(nb=nb, n=n, nrhs=nrhs)
+    ncores, _ = orc.host_cores()
+    out["other_workloads"] = [orc.bench_mult_and_lu(1)]
+    if ncores > 1:
+        out["other_workloads"].append(orc.bench_mult_and_lu(ncores))
+    return out
 
 
 def _release():

@@ -320,7 +320,7 @@ int mops_d_dinverse_trm(int flags, int N, int nb, double *A, int lda, void *stre
 
 /* No counterpart in the reference: for callers that want the LU solve of a
  * well-conditioned system faster than the FP64 factorization allows. */
-enum { MOPS_PREC_FP64 = 0, MOPS_PREC_FP32 = 1 };
+enum { MOPS_PREC_FP64 = 0, MOPS_PREC_FP32 = 1, MOPS_PREC_BF16 = 2 };
 
 typedef struct {
   int iterations;        /* solves with the factors (1 = no refinement step) */
@@ -333,7 +333,9 @@ typedef struct {
 
 /* X (N x nrhs) = A^-1 B: LU with partial pivoting in precision prec
  * (MOPS_PREC_FP32: FP32 trailing updates on the FP32 matrix cores, FP64
- * panels; MOPS_PREC_FP64: the DecomposeLU path), then FP64 refinement
+ * panels; MOPS_PREC_BF16: the same with the trailing products on BF16
+ * operands, U12 scaled by a power of two per panel, FP32 accumulation and
+ * storage; MOPS_PREC_FP64: the DecomposeLU path), then FP64 refinement
  * X += solve(B - A X) until backward_error <= tol or maxit solves. A and B
  * are not modified. Returns 0 even when not converged (see rep->converged).
  * Host memory, synchronous. */
@@ -343,6 +345,15 @@ int mops_dgesv_ir(const mops_mdata_t *A, const mops_mdata_t *B, mops_mdata_t *X,
  * stream once per refinement step). */
 int mops_d_dgesv_ir(int N, int nrhs, const double *A, int lda, const double *B, int ldb, double *X,
                     int ldx, int prec, double tol, int maxit, mops_ir_report *rep, void *stream);
+
+/* The BF16 product of the mixed LU on its own (no counterpart in the
+ * reference; for callers that check it): C (M x N, FP32) += alpha * A * B
+ * with A (M x K) and B (K x N) FP32 in memory, rounded to BF16 (A after a
+ * power-of-two scaling that brings max |A| into [1/2, 1), undone in the
+ * result), products on the BF16 matrix cores with FP32 accumulation.
+ * Device pointers, asynchronous on stream. */
+int mops_d_gemm_bf16(int M, int N, int K, float alpha, const float *A, int lda, const float *B,
+                     int ldb, float *C, int ldc, void *stream);
 
 #ifdef __cplusplus
 }

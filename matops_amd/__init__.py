@@ -7,7 +7,7 @@ the host-side mirror of the reference's Go API and the ctypes binding.
 from .matops import (  # noqa: F401
     LEFT, LOWER, NONE, NOTRANS, RIGHT, TRANS, TRANSA, TRANSB, UNIT, UPPER,
     BlockingParams, CholeskyAdjoint, DecomposeCHOL, DecomposeLU, MatopsError, Mult, NumWorkers,
-    PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
+    PREC_BF16, PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
     blockIndex4, divideWork, scheduleWork, MultSym, MultTrm, RankUpdate2Sym, UpdateTrm,
     DecomposeLUnoPiv, DecomposeLDLnoPiv, SolveLDLnoPiv, InverseTrm,
 )

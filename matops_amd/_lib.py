@@ -39,6 +39,7 @@ SIGNATURES = {
     "mops_trace_collect": (_I, [_I, _P, _P, _P]),
     "mops_dmult_mm_blocked3": (_I, [_PM, _PM, _PM, _D, _D, _I, _I, _I, _I, _I, _I, _I, _I, _I]),
     "mops_d_dgemm": (_I, [_I, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
+    "mops_d_gemm_bf16": (_I, [_I, _I, _I, ctypes.c_float, _P, _I, _P, _I, _P, _I, _P]),
     "mops_d_dgemm_noalloc": (_I, [_I, _I, _I, _I, _D, _P, _I, _P, _I, _D, _P, _I, _P]),
     "mops_dmmat_solve_blk": (_I, [_PM, _PM, _D, _I, _I, _I, _I, _I]),
     "mops_dmmat_solve_unb": (_I, [_PM, _PM, _D, _I, _I, _I, _I]),

@@ -272,8 +272,9 @@ int mops_d_dgesv_ir(int N, int nrhs, const double* A, int lda, const double* B, 
   if (lda < (N > 1 ? N : 1)) return -4;
   if (ldb < (N > 1 ? N : 1)) return -6;
   if (ldx < (N > 1 ? N : 1)) return -8;
-  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32) {
-    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32)", prec);
+  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32 && prec != MOPS_PREC_BF16) {
+    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32, MOPS_PREC_BF16)",
+              prec);
     return -9;
   }
   if (!rep) return -12;
@@ -290,8 +291,9 @@ int mops_dgesv_ir(const mops_mdata_t* A, const mops_mdata_t* B, mops_mdata_t* X,
   if (!X || (N > 0 && nrhs > 0 && !X->md)) return -3;
   if (N < 0) return -4;
   if (nrhs < 0) return -5;
-  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32) {
-    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32)", prec);
+  if (prec != MOPS_PREC_FP64 && prec != MOPS_PREC_FP32 && prec != MOPS_PREC_BF16) {
+    set_error("precision %d is not available (MOPS_PREC_FP64, MOPS_PREC_FP32, MOPS_PREC_BF16)",
+              prec);
     return -6;
   }
   if (!rep) return -9;

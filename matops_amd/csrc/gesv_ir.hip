@@ -2,6 +2,8 @@
 // (no counterpart in matops; it reuses the LU path's kernels).
 //
 //   factor:  prec FP32 -> getrf_mixed_d (FP32 trailing updates, FP64 panels)
+//            prec BF16 -> getrf_mixed_d with the trailing products on BF16
+//                         operands (FP32 accumulation, FP32 storage)
 //            prec FP64 -> getrf_d (the plain FP64 path; one solve suffices)
 //   loop:    D = solve(LU, R)   (FP64 triangular solves on the factors)
 //            X += D
@@ -120,7 +122,7 @@ size_t gesv_ir_ws_bytes(int n, int nrhs, int prec) {
   b += 2 * (size_t)n * nrhs * sizeof(double);         // R, D
   b += (size_t)n * sizeof(int) + 4 * nrhs * sizeof(double) + 4096;
   b += (size_t)NORM_CHUNKS * n * sizeof(double);     // row-sum chunks of the norm
-  const size_t f = prec == 1 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
+  const size_t f = prec != 0 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
   const size_t r = getrs_ws_bytes(n, nrhs) + gemm_split_ws_bytes(n, nrhs, n, sizeof(double));
   return b + (f > r ? f : r);
 }
@@ -133,8 +135,8 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
     res->converged = 1;
     return ST_OK;
   }
-  if (prec != 0 && prec != 1) {
-    set_error("gesv_ir: precision %d is not available (0 = FP64, 1 = FP32)", prec);
+  if (prec != 0 && prec != 1 && prec != 2) {
+    set_error("gesv_ir: precision %d is not available (0 = FP64, 1 = FP32, 2 = BF16)", prec);
     return ST_BADARG;
   }
   if (maxit < 1) maxit = 1;
@@ -164,8 +166,8 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
   MOPS_CHECK_HIP(hipEventRecord(e0, s));
   {
     const size_t mark = ws_mark();
-    if (prec == 1) {
-      MOPS_TRY(getrf_mixed_d(s, n, 256, A, lda, LU, n, nullptr, perm, err));
+    if (prec == 1 || prec == 2) {
+      MOPS_TRY(getrf_mixed_d(s, n, 256, A, lda, LU, n, nullptr, perm, err, prec == 2));
     } else {
       MOPS_TRY(copy_tile_d(s, n, n, A, lda, LU, n));
       MOPS_TRY(getrf_d(s, n, n, 256, LU, n, nullptr, perm, err));

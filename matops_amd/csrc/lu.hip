@@ -903,6 +903,8 @@ int getrf_d(hipStream_t s, int m, int n, int nb, double* A, long lda, long long*
 namespace {
 
 struct MixedCtx {
+  bool bf16;         // trailing products A22 -= L21 U12 on the BF16 matrix cores
+  unsigned* amax[2];  // max |U12| words of the BF16 products, one per stream
   float* T32;        // n x n, T32[c + r*n] = A[r][c]
   float* linv32[LU_NPAR];  // FP32 copies of the panels' unit-triangle inverses
   float* tmp32[2];   // U12 staging for the two streams
@@ -938,8 +940,12 @@ int update_cols32(hipStream_t s, LuCtx& c, MixedCtx& x, int k, int kb, int par, 
   const int below = c.m - k - kb;
   if (below <= 0) return ST_OK;
   if (gemm_trace) trace_mark_begin(s, TR_GEMM_UPDATE);
-  MOPS_TRY(gemm_s(s, false, false, nc, below, kb, -1.0f, U, ld, T + k + (long)(k + kb) * ld, ld,
-                  1.0f, T + j0 + (long)(k + kb) * ld, ld));
+  if (x.bf16)
+    MOPS_TRY(gemm_bf16_s(s, nc, below, kb, -1.0f, U, ld, T + k + (long)(k + kb) * ld, ld,
+                         T + j0 + (long)(k + kb) * ld, ld, x.amax[gemm_trace ? 0 : 1]));
+  else
+    MOPS_TRY(gemm_s(s, false, false, nc, below, kb, -1.0f, U, ld, T + k + (long)(k + kb) * ld,
+                    ld, 1.0f, T + j0 + (long)(k + kb) * ld, ld));
   if (gemm_trace) trace_mark_end(s, TR_GEMM_UPDATE, 2.0 * nc * (double)below * kb);
   return ST_OK;
 }
@@ -974,12 +980,12 @@ size_t getrf_mixed_ws_bytes(int n, int nb) {
   b += (size_t)LU_NPAR * (4 * nb + 2) * sizeof(int);           // pairs
   b += (size_t)LU_NPAR * nb * nb * (sizeof(double) + sizeof(float));  // linv
   b += (size_t)2 * n * nb * sizeof(float);                     // tmp32
-  b += trtri_ws_bytes(nb) + 32 * 256;
+  b += trtri_ws_bytes(nb) + 34 * 256;
   return b;
 }
 
 int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, double* LU, long ldlu,
-                  long long* ipiv, int* perm_out, int* err_flag) {
+                  long long* ipiv, int* perm_out, int* err_flag, bool bf16) {
   if (n <= 0) return ST_OK;
   nb = lu_panel_width(nb, n, LU_NB_MIXED);
   const int G = ceil_div(n, LEAF_R);
@@ -1015,8 +1021,11 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   }
   c.ctr = (unsigned*)ws_carve(256);
   c.err = err_flag ? err_flag : (int*)ws_carve(256);
+  x.bf16 = bf16;
+  x.amax[0] = (unsigned*)ws_carve(256);
+  x.amax[1] = (unsigned*)ws_carve(256);
   if (!x.T32 || !x.P64 || !c.xa || !c.stage || !c.piv || !c.perm || !c.inv0 ||
-      !slots_ok || !x.tmp32[0] || !x.tmp32[1] || !c.ctr || !c.err) {
+      !slots_ok || !x.tmp32[0] || !x.tmp32[1] || !c.ctr || !c.err || !x.amax[0] || !x.amax[1]) {
     set_error("getrf_mixed: workspace too small");
     return ST_NOMEM;
   }

@@ -45,6 +45,14 @@ int gemm_s(hipStream_t s, bool ta, bool tb, int M, int N, int K, float alpha,
            const float* A, long lda, const float* B, long ldb, float beta,
            float* C, long ldc, int mode = GM_FULL, int tri = TRI_NONE);
 
+// C (M x N, FP32) += alpha * A (M x K, FP32) * B (K x N, FP32), the
+// operands rounded to BF16 (A first scaled by the power of two that brings
+// max |A| into [1/2, 1)), products on the BF16 matrix cores with FP32
+// accumulation. amax_word: a device word the call uses for max |A| (calls
+// on one stream may share it).
+int gemm_bf16_s(hipStream_t s, int M, int N, int K, float alpha, const float* A, long lda,
+                const float* B, long ldb, float* C, long ldc, unsigned* amax_word);
+
 // Batched form: entry b uses A + b*sA, B + b*sB, C + b*sC (full C only).
 int gemm_batched_d(hipStream_t s, bool ta, bool tb, int M, int N, int K, double alpha,
                    const double* A, long lda, long sA, const double* B, long ldb, long sB,

@@ -22,10 +22,12 @@ int getrs_d(hipStream_t s, int flags, int n, int nrhs, const double* A, long lda
 // Mixed precision LU of the n x n matrix A (read only): the trailing matrix
 // in FP32 (FP32 matrix cores), each panel widened to FP64 and factored by the
 // FP64 panel code. LU (n x n, FP64) receives the factors (their FP32
-// values), perm_out the row permutation, ipiv the pivots.
+// values), perm_out the row permutation, ipiv the pivots. bf16: the
+// trailing products A22 -= L21 U12 take BF16 operands (U12 scaled by a
+// power of two per panel) on the BF16 matrix cores, accumulating in FP32.
 size_t getrf_mixed_ws_bytes(int n, int nb);
 int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, double* LU, long ldlu,
-                  long long* ipiv, int* perm_out, int* err_flag);
+                  long long* ipiv, int* perm_out, int* err_flag, bool bf16 = false);
 
 }  // namespace mops
 

@@ -372,6 +372,7 @@ def CholeskyAdjoint(L, Lbar, nb=64):
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_BF16 = 2
 
 
 def SolveIR(X, A, B, prec=PREC_FP32, tol=1e-12, maxit=10):

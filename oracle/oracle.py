@@ -231,35 +231,116 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
-def bench_cholesky_sample(nb=256, n=5120, nrhs=64, seed=3):
-    """Times DecomposeCHOL(A, LOWER, nb) + SolveCHOL (nrhs RHS) once on one
-    host core, on an n x n sample of the same kind of input as the GPU
-    workload (A = G G^T / n + I). Uses the reference's calgo kernels when
-    they were built, else the restated ones. Returns the cpu_baseline
-    record (TFLOP/s with the same LAPACK flop count as the GPU figure)."""
+def _flush_cache():
+    """Touches 256 MiB so the next timed run starts with cold caches (the
+    reference's perf mains call mperf.FlushCache() before each run,
+    test/perf_lu.go:188-201)."""
+    buf = np.ones(32 << 20)
+    buf *= 1.0001
+    return float(buf[::4096].sum())
+
+
+def host_cores():
+    """Cores the host offers this process (affinity), and the CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return n, _cpu_model()
+
+
+def _timed(fn, runs=1):
+    """min over `runs` of fn()'s wall time (CLOCK_MONOTONIC), cache flushed
+    before each run."""
+    best = None
+    for _ in range(runs):
+        _flush_cache()
+        t0 = lib().o_now()
+        fn()
+        t = lib().o_now() - t0
+        best = t if best is None else min(best, t)
+    return best
+
+
+def bench_cholesky_sample(nb=256, n=8192, nrhs=64, seed=3):
+    """Times DecomposeCHOL(A, LOWER, nb) + SolveCHOL (nrhs RHS) once, on one
+    host core (the reference's Cholesky runs its kernels serially: only Mult
+    splits over workers, and blockedCHOL calls SolveTrm / RankUpdateSym), on
+    an n x n input of the GPU workload's kind (A = G G^T / n + I). Uses the
+    reference's calgo kernels when they were built, else the restated ones.
+    Returns the cpu_baseline record (TFLOP/s with the same LAPACK flop count
+    as the GPU figure)."""
     kind = "reference" if reference_available() else "port"
     use(kind)
     NumWorkers(1)
     g = np.random.Generator(np.random.Philox(seed))
     G = g.standard_normal((n, n))
     A0 = np.asfortranarray(G @ G.T / n + np.eye(n))
+    del G
     A0 = np.asfortranarray(np.tril(A0) + np.tril(A0, -1).T)
     B0 = np.asfortranarray(g.standard_normal((n, nrhs)))
     A = A0.copy(order="F")
     B = B0.copy(order="F")
-    t0 = lib().o_now()
-    rc, _, _ = DecomposeCHOL(A, LOWER, nb)
-    SolveCHOL(B, A, LOWER)
-    t1 = lib().o_now()
-    if rc != 0:
+    rcs = []
+
+    def run():
+        rcs.append(DecomposeCHOL(A, LOWER, nb)[0])
+        SolveCHOL(B, A, LOWER)
+
+    t = _timed(run)
+    if rcs != [0]:
         raise RuntimeError("oracle Cholesky failed on the sample")
     flops = n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0 + 2.0 * n * n * nrhs
+    ncores, model = host_cores()
     return {
-        "value": round(flops / (t1 - t0) / 1e12, 6),
+        "value": round(flops / t / 1e12, 6),
         "unit": "TFLOP/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"DecomposeCHOL(A, LOWER, nb={nb}) + SolveCHOL, n={n}, {nrhs} RHS, "
-                  f"one run, {t1 - t0:.2f} s on {_cpu_model()}",
-        "seconds": round(t1 - t0, 3),
+        "sample": f"DecomposeCHOL(A, LOWER, nb={nb}) + SolveCHOL, n={n}, {nrhs} RHS (the full "
+                  f"workload), one run after a cache flush, {t:.2f} s on {model} "
+                  f"({ncores} cores available)",
+        "seconds": round(t, 3),
+        "nproc": ncores,
+        "cpu_model": model,
+    }
+
+
+def bench_mult_and_lu(workers, n_mult=1024, n_lu=4096, nb=256, nrhs=16, mult_runs=5):
+    """CPU legs of the other workloads with `workers` Mult workers
+    (NumWorkers, mmult.go:69-73; the reference splits only Mult over them):
+    DGEMM Mult(C, A, B, 1, 0) at n_mult^3 (min of mult_runs, cache flushed),
+    and DecomposeLU(A, piv, nb) + SolveLU with nrhs RHS at n_lu (one run), A
+    of N(0,1) entries. Returns {"dgemm": {...}, "lu": {...}} in TFLOP/s."""
+    kind = "reference" if reference_available() else "port"
+    use(kind)
+    old = NumWorkers(workers)
+    try:
+        g = np.random.Generator(np.random.Philox(1))
+        A = np.asfortranarray(g.uniform(-1, 1, (n_mult, n_mult)))
+        B = np.asfortranarray(g.uniform(-1, 1, (n_mult, n_mult)))
+        C = np.zeros((n_mult, n_mult), order="F")
+        tm = _timed(lambda: Mult(C, A, B, 1.0, 0.0, 0), runs=mult_runs)
+        g = np.random.Generator(np.random.Philox(5))
+        L0 = np.asfortranarray(g.standard_normal((n_lu, n_lu)))
+        X0 = np.asfortranarray(g.standard_normal((n_lu, nrhs)))
+        F = L0.copy(order="F")
+        X = X0.copy(order="F")
+        piv = np.zeros(n_lu, dtype=np.int64)
+
+        def lu():
+            DecomposeLU(F, piv, nb)
+            SolveLU(X, F, piv, 0)
+
+        tl = _timed(lu)
+    finally:
+        NumWorkers(old)
+    lu_flops = 2 * n_lu ** 3 / 3 - n_lu ** 2 / 2 + 5 * n_lu / 6 + (2 * n_lu ** 2 - n_lu) * nrhs
+    return {
+        "dgemm": {"n": n_mult, "seconds": round(tm, 4),
+                  "tflops": round(2 * n_mult ** 3 / tm / 1e12, 6), "runs": mult_runs},
+        "lu": {"n": n_lu, "nb": nb, "nrhs": nrhs, "seconds": round(tl, 3),
+               "tflops": round(lu_flops / tl / 1e12, 6)},
+        "workers": workers,
+        "kind": kind,
     }

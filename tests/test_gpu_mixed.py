@@ -72,8 +72,9 @@ def test_ill_conditioned_reports_without_pretending():
 def test_unknown_precision_is_an_error():
     A, B = _problem(10, 1, 1)
     X = np.zeros_like(B, order="F")
-    with pytest.raises(Exception):
-        mo.SolveIR(X, A, B, 2, 1e-12, 5)
+    for prec in (3, -1):
+        with pytest.raises(Exception):
+            mo.SolveIR(X, A, B, prec, 1e-12, 5)
 
 
 def test_device_api_full_size():
