@@ -17,7 +17,7 @@
 // the panel is a contiguous run. A copy-transpose brings A in and out.
 //
 // Panel (columns k..k+kb, rows k..m-1), right-looking in 32-column leaves.
-// A leaf is one cooperative kernel: G workgroups of 512 threads, one row
+// A leaf is one kernel of G co-resident workgroups of 512 threads, one row
 // per thread held in registers. Per column every workgroup finds its best
 // candidate (wave shuffles) and publishes it with the candidate row into
 // exchange records that start out as a sentinel; every workgroup polls the
@@ -125,8 +125,8 @@ struct LeafArgs {
   int* err;        // set to 1 if a barrier wait timed out
 };
 
-// Grid barrier among the workgroups of a cooperative launch (the leaf's
-// final gather). Everything the workgroups exchange inside the leaf is
+// Grid barrier among the workgroups of one leaf launch (the leaf's final
+// gather). Everything the workgroups exchange inside the leaf is
 // written and read with agent-scope atomics (coherent without cache
 // maintenance), so the barrier only has to
 // order those accesses: every thread's stores are complete (waitcnt) before
@@ -610,6 +610,27 @@ struct LuCtx {
   int* err;
 };
 
+// Whether G workgroups of the leaf kernel fit on the current device at once
+// (with nothing else running), cached per device.
+int leaf_capacity_ok(int G) {
+  static int cap[16] = {0};
+  int d = 0;
+  MOPS_CHECK_HIP(hipGetDevice(&d));
+  if (d < 0 || d >= 16) d = 0;
+  if (cap[d] == 0) {
+    int per_cu = 0, ncu = 0;
+    MOPS_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_getf2_leaf, LEAF_R, 0));
+    MOPS_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d));
+    cap[d] = per_cu * ncu > 0 ? per_cu * ncu : -1;
+  }
+  if (cap[d] < G) {
+    set_error("getrf: %d leaf workgroups do not fit on the device at once (capacity %d)", G,
+              cap[d]);
+    return ST_HIP;
+  }
+  return ST_OK;
+}
+
 int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   const int M = c.m - c0;
   const int G = ceil_div(M, LEAF_R);
@@ -637,9 +658,17 @@ int leaf(hipStream_t s, LuCtx& c, int c0, int w, int pc0, int pc1) {
   a.ctr = c.ctr;
   a.base = c.sync_base;
   a.err = c.err;
-  void* args[] = {&a};
-  MOPS_CHECK_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_leaf, dim3(G), dim3(LEAF_R),
-                                            args, 0, s));
+  // The leaf's workgroups wait for each other (exchange records, one grid
+  // barrier), so all G of them must be resident at once. G <= LEAF_GMAX is
+  // far below the device's capacity for this kernel (checked once per
+  // process against the occupancy the kernel's resources allow), the
+  // kernels running beside a leaf (GEMM tiles) never wait, so every
+  // workgroup of the leaf is dispatched as they drain, and every wait is
+  // bounded: a plain launch, which unlike a cooperative one does not
+  // serialize with the trailing updates on the other streams.
+  MOPS_TRY(leaf_capacity_ok(G));
+  hipLaunchKernelGGL(k_getf2_leaf, dim3(G), dim3(LEAF_R), 0, s, a);
+  MOPS_CHECK_HIP(hipGetLastError());
   // one grid barrier, for the gather, when the panel is wider than the leaf
   const int nsync = (pc1 - pc0) - w > 0 ? 1 : 0;
   c.sync_base += (unsigned)nsync * (unsigned)G;
