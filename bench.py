@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--kernel-stats", metavar="DB",
                     help="print the per-kernel duration table of a rocprofv3 --kernel-trace "
                          "database and exit")
+    ap.add_argument("--counter-table", nargs="+", metavar="DB",
+                    help="print, per kernel, the dispatch count and the average of every "
+                         "counter in rocprofv3 --pmc databases, and exit")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="turn two rocprofv3 --pmc databases of a --profile-run (FETCH_SIZE, "
                          "then WRITE_SIZE) into profiles/traffic.json and exit")
@@ -72,6 +75,8 @@ def main():
         return traffic_summary(*args.traffic_from)
     if args.kernel_stats:
         return kernel_stats(args.kernel_stats)
+    if args.counter_table:
+        return counter_table(args.counter_table)
     if args.profile_run:
         args.steps, args.warmup, args.no_cpu, args.no_extra = 1, 1, True, True
     if args.gpus != 1:
@@ -445,6 +450,25 @@ def kernel_stats(db):
             short = short[:107] + "..."
         print(f"{cnt:6d} {sm / 1e3:12.1f} {av / 1e3:10.2f} {mn / 1e3:10.2f} {mx / 1e3:10.2f} "
               f"{100 * sm / tot:6.2f}  {short}")
+    return 0
+
+
+def counter_table(dbs):
+    """Per kernel and counter: dispatches, average and total value, from the
+    counters_collection view of rocprofv3 --pmc databases."""
+    import sqlite3
+    print(f"{'counter':28} {'calls':>6} {'average':>16} {'total':>18}  kernel")
+    for db in dbs:
+        con = sqlite3.connect(db)
+        rows = con.execute(
+            "select counter_name, kernel_name, count(*), avg(value), sum(value) "
+            "from counters_collection group by counter_name, kernel_name "
+            "order by counter_name, sum(value) desc").fetchall()
+        for cname, kname, cnt, av, tot in rows:
+            short = kname.replace("mops::(anonymous namespace)::", "")
+            if len(short) > 90:
+                short = short[:87] + "..."
+            print(f"{cname:28} {cnt:6d} {av:16.1f} {tot:18.1f}  {short}")
     return 0
 
 
