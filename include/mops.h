@@ -242,9 +242,12 @@ int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *
  * L: the lower Cholesky factor (N x N, lower triangle read). Lbar: on entry
  * d loss / dL in its lower triangle (the upper triangle is ignored); on
  * return the symmetric d loss / dA (both triangles written), so that for a
- * symmetric perturbation E, d loss = sum_ij Abar_ij E_ij. Blocked reverse
- * sweep (Murray 2016) with diagonal blocks of order nb (0 or > 64 -> 64).
- * Uses L, the Lbar buffer and O(64 N) workspace. Host memory, synchronous. */
+ * symmetric perturbation E, d loss = sum_ij Abar_ij E_ij. Closed form
+ * (Murray 2016, eq. 10): Abar = (X + X^T) / 2, X = W^T Phi(L^T Lbar) W with
+ * W = L^-1, on triangle-aware GPU products (5 N^3 / 3 flops). nb >= 0 is
+ * validated and otherwise unused (the closed form has no block size). Uses
+ * L, the Lbar buffer and 3 N^2 doubles of workspace. Host memory,
+ * synchronous. */
 int mops_dpotrf_adjoint(const mops_mdata_t *L, mops_mdata_t *Lbar, int N, int nb);
 /* Device-resident twin: Abar holds Lbar on entry. */
 int mops_d_dpotrf_adjoint(int N, int nb, const double *L, int ldl, double *Abar, int lda,

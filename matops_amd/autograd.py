@@ -30,7 +30,7 @@ class CholeskyFunction(torch.autograd.Function):
     """A (symmetric, lower triangle read) -> L = chol(A), lower.
 
     Forward: DecomposeCHOL(A, LOWER, nb) on the GPU (mops_d_dpotrf).
-    Backward: the blocked reverse sweep (mops_d_dpotrf_adjoint); the
+    Backward: the closed-form adjoint (mops_d_dpotrf_adjoint); the
     gradient returned is the symmetric d loss / dA."""
 
     @staticmethod

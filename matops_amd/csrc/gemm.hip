@@ -67,8 +67,10 @@ struct GemmArgs {
   // chunks before the last leave their partial tiles in part and raise
   // kflags; the last chunk adds them in chunk order and writes C
   int kchunk;
+  int order;  // TILE_* visiting order of the output tiles
   T* part;
   int* kflags;
+  int kready;   // value of a set flag in this launch
   int* status;  // raised (STATUS_GEMM_WAIT) when a wait for a chunk times out
   unsigned spin_limit;  // polls of one chunk flag before giving up
 };
@@ -81,6 +83,40 @@ constexpr int SMALL_KS = 2;  // k shares of the 64x64-tile kernel for full outpu
 // 7.72 ms (16).
 constexpr int SG = 4;
 constexpr int NXCD = 8;
+
+// Visiting order of the output tiles. TILE_GROUPED: groups of GROUP_M tile
+// rows, renumbered so that each XCD gets a contiguous range (L2 reuse of
+// the row panels). The others serve products whose tiles have very
+// different k ranges (triangular operands): heaviest tiles first, and the
+// ids go to the XCDs round robin as the hardware hands them out, so every
+// XCD gets its share of heavy and light tiles.
+//   TILE_ROWS_UP / _DOWN     row by row from the top / bottom (work
+//                            depends on the tile row)
+//   TILE_COLS_LEFT / _RIGHT  column by column from the left / right
+//   TILE_SHELLS              square grids: shells d = max(tm, tn) = 0, 1, ...
+//                            (work decreasing in max(tm, tn))
+//   TILE_LOWER_ROWS          lower triangle tm >= tn, row by row from the
+//                            top (work decreasing in tm)
+//   TILE_LOWER_BANDS         lower triangle by diagonals tm - tn from the
+//                            farthest (work increasing in tm - tn)
+enum : int {
+  TILE_GROUPED = 0,
+  TILE_ROWS_UP = 1,
+  TILE_ROWS_DOWN = 2,
+  TILE_COLS_LEFT = 3,
+  TILE_COLS_RIGHT = 4,
+  TILE_SHELLS = 5,
+  TILE_LOWER_ROWS = 6,
+  TILE_LOWER_BANDS = 7
+};
+
+// r with r (r + 1) / 2 <= x < (r + 1) (r + 2) / 2
+__device__ __forceinline__ int tri_root(int x) {
+  long r = (long)((sqrtf(8.0f * (float)x + 1.0f) - 1.0f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= x) ++r;
+  while (r * (r + 1) / 2 > x) --r;
+  return (int)r;
+}
 
 // Hardware hands workgroup ids to the XCDs round robin (id % 8). Renumber so
 // that XCD x sees a contiguous range of logical tile indices.
@@ -136,9 +172,44 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
   T(*Bs)[BK * LDS_B] = reinterpret_cast<T(*)[BK * LDS_B]>(smem + 2 * BK * LDS_A);
 
   // ---- tile selection ----
-  const int lin = xcd_linear(blockIdx.x, gridDim.x);
+  const int lin = g.order == TILE_GROUPED ? xcd_linear(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   int tm, tn;
-  if (g.mode == GM_FULL || g.mode == GM_LOWER_T || g.mode == GM_UPPER_T) {
+  if (g.order != TILE_GROUPED) {
+    const int Tm = g.tiles_m, Tn = g.tiles_n;
+    switch (g.order) {
+      case TILE_ROWS_UP: tm = lin / Tn; tn = lin % Tn; break;
+      case TILE_ROWS_DOWN: tm = Tm - 1 - lin / Tn; tn = lin % Tn; break;
+      case TILE_COLS_LEFT: tn = lin / Tm; tm = lin % Tm; break;
+      case TILE_COLS_RIGHT: tn = Tn - 1 - lin / Tm; tm = lin % Tm; break;
+      case TILE_SHELLS: {
+        long d = (long)sqrtf((float)lin);
+        while ((d + 1) * (d + 1) <= lin) ++d;
+        while (d * d > lin) --d;
+        const int o = (int)(lin - d * d);  // shell d: (d, 0..d), then (0..d-1, d)
+        if (o <= d) {
+          tm = (int)d;
+          tn = o;
+        } else {
+          tm = o - (int)d - 1;
+          tn = (int)d;
+        }
+        break;
+      }
+      case TILE_LOWER_ROWS: {
+        const int r = tri_root(lin);
+        tm = r;
+        tn = (int)(lin - (long)r * (r + 1) / 2);
+        break;
+      }
+      default: {  // TILE_LOWER_BANDS: band r holds the r + 1 tiles of tm - tn = Tm - 1 - r
+        const int r = tri_root(lin);
+        tn = (int)(lin - (long)r * (r + 1) / 2);
+        tm = tn + Tm - 1 - r;
+        break;
+      }
+    }
+    if (tm >= Tm || tn >= Tn) return;
+  } else if (g.mode == GM_FULL || g.mode == GM_LOWER_T || g.mode == GM_UPPER_T) {
     const int per_group = GROUP_M * g.tiles_n;
     const int grp = lin / per_group;
     const int first = grp * GROUP_M;
@@ -177,6 +248,11 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     case TRI_A_UPPER: kbeg = m0; break;
     case TRI_B_LOWER: kbeg = n0; break;
     case TRI_B_UPPER: kend = min(g.K, n0 + BN); break;
+    case TRI_AB_LOWER:
+      kbeg = n0;
+      kend = min(g.K, m0 + BM);
+      break;
+    case TRI_AU_BL: kbeg = max(m0, n0); break;
     default: break;
   }
   kbeg = (kbeg / BK) * BK;
@@ -383,7 +459,7 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0)
-        __hip_atomic_store(&g.kflags[tid_tile * S + z], 1, __ATOMIC_RELEASE,
+        __hip_atomic_store(&g.kflags[tid_tile * S + z], g.kready, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
       return;
     }
@@ -396,7 +472,7 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
             break;
           }
           if (__hip_atomic_load(&g.kflags[tid_tile * S + zz], __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT) != 0)
+                                __HIP_MEMORY_SCOPE_AGENT) == g.kready)
             break;
           __builtin_amdgcn_s_sleep(1);
         }
@@ -492,12 +568,16 @@ inline int grid_for(long total) {
   return (int)b;
 }
 
-// Scratch of a split-k launch: partial tiles and chunk flags. Carved from
-// the calling thread's workspace, or taken from the buffer of a
-// SplitScratchOverride in scope. False when it does not fit.
+// Scratch of a split-k launch: partial tiles and chunk flags. The partial
+// tiles are carved from the calling thread's workspace and the flags taken
+// from its flag pool (ready marks a set flag; nothing to clear); with a
+// SplitScratchOverride in scope both come from its buffer, whose flags the
+// launch clears first (ready = 1, clear = true). False when they do not fit.
 struct SplitScratch {
   void* part;
   int* flags;
+  int ready;
+  bool clear;
 };
 thread_local void* g_split_base = nullptr;
 thread_local size_t g_split_bytes = 0;
@@ -508,19 +588,53 @@ bool split_scratch(long part_bytes, long nflags, SplitScratch& sc) {
     if (pb + fb > g_split_bytes) return false;
     sc.part = g_split_base;
     sc.flags = (int*)((char*)g_split_base + pb);
+    sc.ready = 1;
+    sc.clear = true;
     return true;
   }
   sc.part = ws_carve(pb);
-  sc.flags = (int*)ws_carve(fb);
-  return sc.part && sc.flags;
+  sc.clear = false;
+  return sc.part && signal_flags(nflags, &sc.flags, &sc.ready);
+}
+
+// Tile order of a product (the k range of a tile follows from the known
+// zeros of its operands, so the heavy tiles sit in known rows, columns or
+// shells).
+int tile_order(int mode, int tri) {
+  if (mode == GM_FULL) {
+    switch (tri) {
+      case TRI_A_LOWER: return TILE_ROWS_DOWN;   // k < m0 + BM
+      case TRI_A_UPPER: return TILE_ROWS_UP;     // k >= m0
+      case TRI_B_LOWER: return TILE_COLS_LEFT;   // k >= n0
+      case TRI_B_UPPER: return TILE_COLS_RIGHT;  // k < n0 + BN
+      case TRI_AU_BL: return TILE_SHELLS;        // k >= max(m0, n0)
+      default: return TILE_GROUPED;
+    }
+  }
+  if (mode == GM_LOWER) {
+    if (tri == TRI_A_UPPER) return TILE_LOWER_ROWS;    // k >= m0
+    if (tri == TRI_AB_LOWER) return TILE_LOWER_BANDS;  // n0 <= k < m0 + BM
+  }
+  return TILE_GROUPED;
 }
 
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, int KS>
 int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int batch2) {
   g.tiles_m = ceil_div(g.M, BM);
   g.tiles_n = ceil_div(g.N, BN);
+  // shells and the lower-triangle orders need a square tile grid
+  if (g.order != TILE_GROUPED && g.order >= TILE_SHELLS && g.tiles_m != g.tiles_n) {
+    if (g.order != TILE_SHELLS) {
+      set_error("gemm: triangle order on a non-square output");
+      return ST_BADARG;
+    }
+    g.order = TILE_GROUPED;
+  }
   long nblk;
-  if (g.mode == GM_FULL || g.mode == GM_LOWER_T || g.mode == GM_UPPER_T) {
+  if (g.order == TILE_LOWER_ROWS || g.order == TILE_LOWER_BANDS) {
+    nblk = (long)g.tiles_m * (g.tiles_m + 1) / 2;
+  } else if (g.order != TILE_GROUPED || g.mode == GM_FULL || g.mode == GM_LOWER_T ||
+             g.mode == GM_UPPER_T) {
     nblk = (long)g.tiles_m * g.tiles_n;
   } else {
     const long ts = ceil_div(g.tiles_m, SG);
@@ -532,7 +646,7 @@ int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int b
     return ST_BADARG;
   }
   dim3 grid((unsigned)nblk, (unsigned)batch, (unsigned)batch2), block(64 * WGM * WGN * KS);
-  const bool tri = g.mode == GM_LOWER || g.mode == GM_UPPER;
+  const bool tri = g.order == TILE_GROUPED && (g.mode == GM_LOWER || g.mode == GM_UPPER);
 #define MOPS_GEMM_LAUNCH(TA_, TB_)                                                              \
   if (tri)                                                                                     \
     hipLaunchKernelGGL((k_gemm<T, BM, BN, BK, WGM, WGN, KS, TA_, TB_, true>), grid, block, 0, s, g); \
@@ -618,8 +732,10 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
   g.sB2 = sB2;
   g.sC2 = sC2;
   g.kchunk = 0;
+  g.order = TILE_GROUPED;
   g.part = nullptr;
   g.kflags = nullptr;
+  g.kready = 0;
   g.status = nullptr;
   g.spin_limit = 0;
   // Few output tiles and a long k: split k so that the grid still covers
@@ -632,8 +748,9 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
     // sides): 128x16 tiles with 32-deep slabs, two k shares and two slabs in
     // flight keep 64 KiB of A in flight per CU (one workgroup per CU); k is
     // split in S2 chunks so that the tm x S2 workgroups fill whole rounds of
-    // the CUs. The plan is made before the flags are cleared, so that
-    // exactly the flags this launch polls start at zero.
+    // the CUs. The plan is made before the flags are taken, so that exactly
+    // the flags this launch polls are taken (and, in a caller's buffer,
+    // cleared).
     int S2 = 0, kc2 = 0;
     if (mode == GM_FULL && gemm_skinny(M, N, K)) {
       int dev = 0, ncu = 0;
@@ -652,11 +769,14 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
     int* status = active_status();
     if (split_scratch(part_elems * (long)sizeof(T), nflags, sc) && status &&
         used_flags <= nflags) {
-      hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(used_flags)), dim3(256), 0, s, sc.flags,
-                         (int)used_flags);
-      MOPS_CHECK_HIP(hipGetLastError());
+      if (sc.clear) {
+        hipLaunchKernelGGL(k_zero_flags_i32, dim3(grid_for(used_flags)), dim3(256), 0, s,
+                           sc.flags, (int)used_flags);
+        MOPS_CHECK_HIP(hipGetLastError());
+      }
       g.part = (T*)sc.part;
       g.kflags = sc.flags;
+      g.kready = sc.ready;
       g.status = status;
       g.spin_limit = wait_spin_limit(1u << 26);
       int st;
@@ -672,6 +792,16 @@ int gemm_impl(hipStream_t s, bool ta, bool tb, int M, int N, int K, T alpha, con
       return st;
     }
     ws_release(mark);
+  }
+  // Products with a triangular operand: heaviest tiles first, spread over
+  // the XCDs. A lower-triangle output with a triangular operand is a
+  // trapezoid launch over the lower tiles (element mask on the diagonal).
+  g.order = tile_order(mode, tri);
+  if (g.order == TILE_LOWER_ROWS || g.order == TILE_LOWER_BANDS) {
+    g.mode = GM_LOWER_T;
+    if (gemm_big_tiles(M, N, (long)batch * batch2))
+      return launch_gemm<T, 128, 128, 16, 2, 4, 1>(s, ta, tb, g, batch, batch2);
+    return launch_gemm<T, 64, 64, 16, 2, 2, SMALL_KS>(s, ta, tb, g, batch, batch2);
   }
   // Large outputs: 128x128 tiles (8 MFMA blocks per wave, 8 waves). Smaller
   // ones, and the triangle-only updates (whose diagonal tiles do wasted

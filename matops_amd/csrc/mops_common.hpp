@@ -110,6 +110,16 @@ int take_status(hipStream_t s);
 // or the pointer installed by a StatusOverride in scope (entry points that
 // may not allocate pass one inside their own scratch).
 int* active_status();
+// Flags of a kernel whose workgroups signal each other (split-k GEMM,
+// chained triangular solve): count ints of the calling thread's flag pool
+// on the current device, and *ready, the value that marks a flag as set in
+// this launch. Every launch gets a value no earlier launch of the thread
+// used, so whatever earlier launches left in the pool (also one whose wait
+// gave up) never reads as set, and no launch has to clear its flags first.
+// Launches take consecutive ranges of the pool, so launches in flight on
+// other streams hold other flags. False when count exceeds the pool (1 Mi
+// flags) or the pool cannot be allocated.
+bool signal_flags(long count, int** flags, int* ready);
 // Status word of the no-allocation GEMM on the current device (inside its
 // scratch array), nullptr until that entry point first ran there.
 int* noalloc_status_word();

@@ -18,7 +18,17 @@ enum : int { GM_FULL = 0, GM_LOWER = 1, GM_UPPER = 2, GM_LOWER_T = 3, GM_UPPER_T
 // slab is never read, a partially covered one is read in full.
 //   TRI_A_LOWER: op(A)[i][k] == 0 for k > i      TRI_A_UPPER: k < i
 //   TRI_B_LOWER: op(B)[k][j] == 0 for k < j      TRI_B_UPPER: k > j
-enum : int { TRI_NONE = 0, TRI_A_LOWER = 1, TRI_A_UPPER = 2, TRI_B_LOWER = 3, TRI_B_UPPER = 4 };
+//   TRI_AB_LOWER: both TRI_A_LOWER and TRI_B_LOWER (j <= k <= i)
+//   TRI_AU_BL:    both TRI_A_UPPER and TRI_B_LOWER (k >= max(i, j))
+enum : int {
+  TRI_NONE = 0,
+  TRI_A_LOWER = 1,
+  TRI_A_UPPER = 2,
+  TRI_B_LOWER = 3,
+  TRI_B_UPPER = 4,
+  TRI_AB_LOWER = 5,
+  TRI_AU_BL = 6
+};
 
 // While one is in scope, the split-k launches of gemm_d / gemm_s on the
 // calling thread take their partial tiles and flags from this device buffer

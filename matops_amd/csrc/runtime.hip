@@ -27,6 +27,9 @@ struct ThreadState {
   bool bulk_tried[16] = {};
   hipEvent_t ev[16][6] = {};
   int* status[16] = {};                 // sticky status words (device ints)
+  int* flags[16] = {};                  // signal flag pools
+  long flags_next[16] = {};             // next free flag of the pool
+  int flags_ready[16] = {};             // ready value of the last launch
   hipEvent_t ws_done[16] = {};          // end of the last call's work
   hipStream_t ws_done_stream[16] = {};  // the stream it was recorded on
   bool ws_done_set[16] = {};
@@ -151,6 +154,44 @@ int* status_word() {
   return g_ts.status[d];
 }
 
+constexpr long FLAG_POOL = 1l << 20;
+
+bool signal_flags(long count, int** flags, int* ready) {
+  if (count <= 0 || count > FLAG_POOL) return false;
+  const int d = current_device();
+  bool clear = false;
+  if (!g_ts.flags[d]) {
+    void* p = nullptr;
+    if (hipMalloc(&p, FLAG_POOL * sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    g_ts.flags[d] = static_cast<int*>(p);
+    clear = true;
+  } else if (g_ts.flags_ready[d] == 0x7fffffff) {
+    // every ready value has been used: earlier ones could read as set again
+    if (hipDeviceSynchronize() != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    clear = true;
+  }
+  if (clear) {
+    // synchronous on purpose: the pool is zero before any kernel can see it
+    if (hipMemset(g_ts.flags[d], 0, FLAG_POOL * sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    g_ts.flags_ready[d] = 0;
+    g_ts.flags_next[d] = 0;
+  }
+  if (g_ts.flags_next[d] + count > FLAG_POOL) g_ts.flags_next[d] = 0;
+  *flags = g_ts.flags[d] + g_ts.flags_next[d];
+  g_ts.flags_next[d] += (count + 63) / 64 * 64;  // 256-byte aligned ranges
+  *ready = ++g_ts.flags_ready[d];
+  return true;
+}
+
 unsigned wait_spin_limit(unsigned dflt) {
   // -1: not read yet, -2: not set
   static std::atomic<long long> g_limit{-1};
@@ -196,7 +237,8 @@ void thread_release() {
   int cur = 0;
   (void)hipGetDevice(&cur);
   for (int d = 0; d < 16; ++d) {
-    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d] && !g_ts.status[d])
+    if (!g_ts.ws[d].base && !g_ts.stream[d] && !g_ts.aux[d] && !g_ts.bulk[d] && !g_ts.status[d] &&
+        !g_ts.flags[d])
       continue;
     if (hipSetDevice(d) != hipSuccess) continue;
     if (g_ts.stream[d]) (void)hipStreamSynchronize(g_ts.stream[d]);
@@ -205,6 +247,10 @@ void thread_release() {
     if (g_ts.ws[d].base) (void)hipFree(g_ts.ws[d].base);
     if (g_ts.status[d]) (void)hipFree(g_ts.status[d]);
     g_ts.status[d] = nullptr;
+    if (g_ts.flags[d]) (void)hipFree(g_ts.flags[d]);
+    g_ts.flags[d] = nullptr;
+    g_ts.flags_next[d] = 0;
+    g_ts.flags_ready[d] = 0;
     if (g_ts.stream[d]) (void)hipStreamDestroy(g_ts.stream[d]);
     if (g_ts.aux[d]) {
       (void)hipStreamSynchronize(g_ts.aux[d]);

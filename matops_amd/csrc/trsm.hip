@@ -108,15 +108,16 @@ __device__ __forceinline__ bool tc_is_sent(double v) {
   return (unsigned long long)__double_as_longlong(v) == TC_SENT;
 }
 
-// Waits until *flag != 0 for at most `limit` polls; then raises the status
-// bit and gives up (limit 0: gives up at once, without polling).
-__device__ __forceinline__ void tc_wait_flag(const int* flag, int* err, unsigned limit) {
+// Waits until *flag == ready for at most `limit` polls; then raises the
+// status bit and gives up (limit 0: gives up at once, without polling).
+__device__ __forceinline__ void tc_wait_flag(const int* flag, int ready, int* err,
+                                             unsigned limit) {
   for (unsigned spins = 0;; ++spins) {
     if (spins >= limit) {
       raise_status(err, STATUS_TRSM_WAIT);
       break;
     }
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ready) break;
     __builtin_amdgcn_s_sleep(1);
   }
 }
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
                                                      const double* __restrict__ A, long lda,
                                                      const double* __restrict__ Dinv, double* B,
                                                      long ldb, double* W, long ldw, int* flags,
-                                                     int* err, unsigned spin_limit) {
+                                                     int ready, int* err, unsigned spin_limit) {
   __shared__ double As[TC_BS * TC_LDA];  // op(A) block [row][k]
   __shared__ double Xs[TC_BS * TC_LDX];  // X block [k][col]
   __shared__ double Ys[TC_BS * TC_LDX];  // acc [k][col]
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
         direct_fetch(blk(it + 1), pre);
     }
     if (!last) {
-      if (t == 0) tc_wait_flag(&flags[c * ngrp + g], err, spin_limit);
+      if (t == 0) tc_wait_flag(&flags[c * ngrp + g], ready, err, spin_limit);
       __syncthreads();
     }
     load_x(c, last);
@@ -336,16 +337,12 @@ __global__ __launch_bounds__(TC_T) void k_trsm_chain(int N, int nrhs, int fwd,
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (t == 0)
-    __hip_atomic_store(&flags[b * ngrp + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&flags[b * ngrp + g], ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, long n, unsigned long long v) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     p[i] = v;
-}
-
-__global__ void k_zero_i32(int* p, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // All nblk blocks (order bb each, same strides) are inverted together.
@@ -480,11 +477,11 @@ size_t trsm_ws_bytes(int flags, int N, int nrhs) {
 
 namespace {
 // Workspace of the chained LEFT solve: the inverses of the 64x64 diagonal
-// blocks, the sentinel-filled copy of the solution and the flags.
+// blocks and the sentinel-filled copy of the solution (the flags come from
+// the thread's flag pool, per solve).
 struct ChainWs {
   double* D;
   double* W;
-  int* fl;
   long ldw;
   int nb64, ngrp;
 };
@@ -495,8 +492,7 @@ int chain_ws(int N, int nrhs, ChainWs& w) {
   w.ldw = (long)w.nb64 * TC_BS;
   w.D = (double*)ws_carve((size_t)w.nb64 * TC_BS * TC_BS * sizeof(double));
   w.W = (double*)ws_carve((size_t)w.ldw * w.ngrp * TC_CG * sizeof(double));
-  w.fl = (int*)ws_carve((size_t)(w.nb64 * w.ngrp + 1) * sizeof(int));
-  if (!w.D || !w.W || !w.fl) {
+  if (!w.D || !w.W) {
     set_error("trsm: workspace too small");
     return ST_NOMEM;
   }
@@ -509,8 +505,12 @@ int chain_solve(hipStream_t s, int flags, int N, int nrhs, const double* A, long
                 long ldb, const ChainWs& w) {
   const bool trans = flags & F_TRANSA;
   const bool eff_lower = ((flags & F_UPPER) != 0) == trans;  // op(A) is lower triangular
-  const int nflag = w.nb64 * w.ngrp + 1;
-  hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(nflag, 256)), dim3(256), 0, s, w.fl, nflag);
+  int* fl = nullptr;
+  int ready = 0;
+  if (!signal_flags((long)w.nb64 * w.ngrp, &fl, &ready)) {
+    set_error("trsm: signal flags could not be allocated");
+    return ST_NOMEM;
+  }
   const long nw = w.ldw * w.ngrp * TC_CG;
   hipLaunchKernelGGL(k_fill_u64, dim3((int)std::min<long>((nw + 255) / 256, 4096)), dim3(256), 0,
                      s, (unsigned long long*)w.W, nw, TC_SENT);
@@ -523,10 +523,10 @@ int chain_solve(hipStream_t s, int flags, int N, int nrhs, const double* A, long
   }
   if (trans)
     hipLaunchKernelGGL(k_trsm_chain<true>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
-                       w.D, B, ldb, w.W, w.ldw, w.fl, err, wait_spin_limit(TC_SPIN_LIMIT));
+                       w.D, B, ldb, w.W, w.ldw, fl, ready, err, wait_spin_limit(TC_SPIN_LIMIT));
   else
     hipLaunchKernelGGL(k_trsm_chain<false>, grid, block, 0, s, N, nrhs, eff_lower ? 1 : 0, A, lda,
-                       w.D, B, ldb, w.W, w.ldw, w.fl, err, wait_spin_limit(TC_SPIN_LIMIT));
+                       w.D, B, ldb, w.W, w.ldw, fl, ready, err, wait_spin_limit(TC_SPIN_LIMIT));
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }

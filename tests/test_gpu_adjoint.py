@@ -1,6 +1,7 @@
 """GPU parity of the Cholesky adjoint (mops_dpotrf_adjoint /
-mops_d_dpotrf_adjoint, blocked reverse sweep): against the oracle's closed
-form, the A = I special case, central finite differences through the
+mops_d_dpotrf_adjoint; the GPU evaluates the closed form with an explicit
+L^-1 and triangle-aware products, the oracle with two triangular solves):
+against the oracle, the A = I special case, central finite differences through the
 Cholesky (h = 1e-6, 20 symmetric directions, unit Frobenius norm), torch's
 own Cholesky backward, and at the full n = 4096 workload
 (A = Q diag(logspace(0, 4)) Q^T, W lower N(0,1)) by finite differences.

@@ -121,3 +121,48 @@ def test_dmult_alpha_zero_propagates_inf_like_the_reference():
     np.testing.assert_array_equal(np.isnan(C), np.isnan(Co))
     assert np.isnan(C[5]).all() and not np.isnan(np.delete(C, 5, axis=0)).any()
     assert np.all(C[~np.isnan(C)] == 0.0)
+
+
+def test_split_flags_reused_across_many_launches():
+    """The chunk flags of split-k launches come from a per-thread pool, and
+    each launch marks them with a value no earlier launch used (nothing is
+    cleared between launches). 1100 launches that take 1024 flags each go
+    around the pool of 2^20 flags once; every result must equal the first
+    (the split adds its partial tiles in a fixed order), alternating with
+    chained triangular solves that take their flags from the same pool."""
+    import torch
+    dev = torch.device("cuda:0")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.lib()
+    g = torch.Generator(device=dev).manual_seed(3)
+    m, k = 1000, 8192  # 16 x 16 tiles of 64, k split in 4 chunks
+    A = torch.randn((k, m), generator=g, device=dev, dtype=torch.float64).T
+    B = torch.randn((m, k), generator=g, device=dev, dtype=torch.float64).T
+    C = torch.empty((m, m), device=dev, dtype=torch.float64).T
+    ref = None
+    n = 512
+    T = torch.tril(torch.randn((n, n), generator=g, device=dev, dtype=torch.float64) * 0.1)
+    T = (T + 3.0 * torch.eye(n, device=dev, dtype=torch.float64)).T.contiguous().T
+    X0 = torch.randn((8, n), generator=g, device=dev, dtype=torch.float64).T
+    X = torch.empty_like(X0.T).T
+    xref = None
+    print("[flags] 1100 split-k GEMMs 1000x1000x8192 with chained solves", flush=True)
+    for it in range(1100):
+        assert L.mops_d_dgemm(0, m, m, k, 1.0, A.data_ptr(), m, B.data_ptr(), k, 0.0,
+                              C.data_ptr(), m, s) == 0
+        if ref is None:
+            ref = C.clone()
+            assert torch.allclose(ref, A @ B, rtol=0, atol=1e-9)
+        elif not torch.equal(C, ref):
+            pytest.fail(f"launch {it}: result differs from the first")
+        if it % 100 == 0:
+            X.copy_(X0)
+            assert L.mops_d_dtrsm(mo.LEFT | mo.LOWER, n, 8, 1.0, T.data_ptr(), n, X.data_ptr(),
+                                  n, s) == 0
+            if xref is None:
+                xref = X.clone()
+                assert torch.allclose(T @ xref, X0, rtol=0, atol=1e-12)
+            else:
+                assert torch.equal(X, xref), it
+    torch.cuda.synchronize()
+    assert L.mops_d_status(s) == 0
