@@ -408,10 +408,10 @@ def extra_mixed(L, ok, colmajor, torch, dev, sp, xref, lu_total_ms, n=16384, nrh
 
 def extra_adjoint(L, ok, colmajor, torch, dev, sp, n=4096):
     """Cholesky adjoint n = 4096: A = Q diag(logspace(0, 4)) Q^T, W lower
-    N(0,1); time of mops_d_dpotrf_adjoint. "tflops" counts the 2n^3/3 flops
-    of the blocked reverse sweep (the cheapest known form, so the rate is
-    comparable across algorithms); "tflops_executed" the 5n^3/3 of the
-    closed form the GPU runs."""
+    N(0,1); time of mops_d_dpotrf_adjoint with its default block order.
+    "tflops" counts the 2n^3/3 flops of the blocked reverse sweep (the
+    cheapest known form, so the rate is comparable across algorithms);
+    "tflops_executed" the 5n^3/3 of the closed form the GPU runs."""
     g = torch.Generator(device="cpu").manual_seed(7)
     X = torch.randn((n, n), dtype=torch.float64, generator=g).to(dev)
     Q, _ = torch.linalg.qr(X)
@@ -428,7 +428,7 @@ def extra_adjoint(L, ok, colmajor, torch, dev, sp, n=4096):
         Ab.copy_(W)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ok(L.mops_d_dpotrf_adjoint(n, 64, Lc.data_ptr(), n, Ab.data_ptr(), n, sp), "adjoint")
+        ok(L.mops_d_dpotrf_adjoint(n, 0, Lc.data_ptr(), n, Ab.data_ptr(), n, sp), "adjoint")
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         best = t if best is None else min(best, t)

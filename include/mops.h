@@ -243,11 +243,11 @@ int mops_d_dpotrs(int flags, int N, int nrhs, const double *A, int lda, double *
  * d loss / dL in its lower triangle (the upper triangle is ignored); on
  * return the symmetric d loss / dA (both triangles written), so that for a
  * symmetric perturbation E, d loss = sum_ij Abar_ij E_ij. Closed form
- * (Murray 2016, eq. 10): Abar = (X + X^T) / 2, X = W^T Phi(L^T Lbar) W with
- * W = L^-1, on triangle-aware GPU products (5 N^3 / 3 flops). nb >= 0 is
- * validated and otherwise unused (the closed form has no block size). Uses
- * L, the Lbar buffer and 3 N^2 doubles of workspace. Host memory,
- * synchronous. */
+ * (Murray 2016, eq. 10): Abar = (X + X^T) / 2, X = L^-T Phi(L^T Lbar) L^-1,
+ * evaluated in place in blocks of order nb (0 -> 512; rounded up to a
+ * multiple of 64, at most 1024): a blocked in-place TRMM and two blocked
+ * in-place TRSMs (5 N^3 / 3 flops). Uses L, the Lbar buffer and O(nb N)
+ * workspace. Host memory, synchronous. */
 int mops_dpotrf_adjoint(const mops_mdata_t *L, mops_mdata_t *Lbar, int N, int nb);
 /* Device-resident twin: Abar holds Lbar on entry. */
 int mops_d_dpotrf_adjoint(int N, int nb, const double *L, int ldl, double *Abar, int lda,

@@ -55,7 +55,7 @@ class CholeskyFunction(torch.autograd.Function):
         n = L.shape[0]
         Lc = _colmajor_copy(L)
         Ab = _colmajor_copy(torch.tril(gL))
-        rc = _lib.lib().mops_d_dpotrf_adjoint(n, 64, Lc.data_ptr(), n, Ab.data_ptr(), n,
+        rc = _lib.lib().mops_d_dpotrf_adjoint(n, 0, Lc.data_ptr(), n, Ab.data_ptr(), n,
                                               _stream())
         _lib.check("mops_d_dpotrf_adjoint", rc)
         return Ab.contiguous(), None

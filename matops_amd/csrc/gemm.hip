@@ -99,6 +99,10 @@ constexpr int NXCD = 8;
 //                            top (work decreasing in tm)
 //   TILE_LOWER_BANDS         lower triangle by diagonals tm - tn from the
 //                            farthest (work increasing in tm - tn)
+//   TILE_LOWER_TRAP          the tiles of a lower trapezoid (GM_LOWER_T)
+//                            that hold elements with row >= column, row by
+//                            row from the top: min(tm + 1, tiles_n) in row
+//                            tm (the other tiles are not launched)
 enum : int {
   TILE_GROUPED = 0,
   TILE_ROWS_UP = 1,
@@ -107,7 +111,8 @@ enum : int {
   TILE_COLS_RIGHT = 4,
   TILE_SHELLS = 5,
   TILE_LOWER_ROWS = 6,
-  TILE_LOWER_BANDS = 7
+  TILE_LOWER_BANDS = 7,
+  TILE_LOWER_TRAP = 8
 };
 
 // r with r (r + 1) / 2 <= x < (r + 1) (r + 2) / 2
@@ -199,6 +204,20 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
         const int r = tri_root(lin);
         tm = r;
         tn = (int)(lin - (long)r * (r + 1) / 2);
+        break;
+      }
+      case TILE_LOWER_TRAP: {
+        // rows 0 .. Tn - 1 hold a triangle of Tn (Tn + 1) / 2 tiles, the rows
+        // below Tn tiles each
+        const long tri = (long)Tn * (Tn + 1) / 2;
+        if (lin < tri) {
+          const int r = tri_root(lin);
+          tm = r;
+          tn = (int)(lin - (long)r * (r + 1) / 2);
+        } else {
+          tm = Tn + (int)((lin - tri) / Tn);
+          tn = (int)((lin - tri) % Tn);
+        }
         break;
       }
       default: {  // TILE_LOWER_BANDS: band r holds the r + 1 tiles of tm - tn = Tm - 1 - r
@@ -623,16 +642,28 @@ int launch_gemm(hipStream_t s, bool ta, bool tb, GemmArgs<T> g, int batch, int b
   g.tiles_m = ceil_div(g.M, BM);
   g.tiles_n = ceil_div(g.N, BN);
   // shells and the lower-triangle orders need a square tile grid
-  if (g.order != TILE_GROUPED && g.order >= TILE_SHELLS && g.tiles_m != g.tiles_n) {
+  if (g.order >= TILE_SHELLS && g.order <= TILE_LOWER_BANDS && g.tiles_m != g.tiles_n) {
     if (g.order != TILE_SHELLS) {
       set_error("gemm: triangle order on a non-square output");
       return ST_BADARG;
     }
     g.order = TILE_GROUPED;
   }
+  // a lower trapezoid whose launch would be mostly tiles above the diagonal
+  // (which exit at once, and all in the first XCDs' share of the grouped
+  // order): launch only the tiles below it
+  if (g.order == TILE_GROUPED && g.mode == GM_LOWER_T) {
+    const long all = (long)g.tiles_m * g.tiles_n;
+    const long tm = g.tiles_m, tn = g.tiles_n;
+    const long work = tm <= tn ? tm * (tm + 1) / 2 : tn * (tn + 1) / 2 + (tm - tn) * tn;
+    if (4 * all > 5 * work) g.order = TILE_LOWER_TRAP;
+  }
   long nblk;
   if (g.order == TILE_LOWER_ROWS || g.order == TILE_LOWER_BANDS) {
     nblk = (long)g.tiles_m * (g.tiles_m + 1) / 2;
+  } else if (g.order == TILE_LOWER_TRAP) {
+    const long tm = g.tiles_m, tn = g.tiles_n;
+    nblk = tm <= tn ? tm * (tm + 1) / 2 : tn * (tn + 1) / 2 + (tm - tn) * tn;
   } else if (g.order != TILE_GROUPED || g.mode == GM_FULL || g.mode == GM_LOWER_T ||
              g.mode == GM_UPPER_T) {
     nblk = (long)g.tiles_m * g.tiles_n;

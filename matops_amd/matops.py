@@ -353,7 +353,7 @@ def SolveCHOL(B, A, flags):
         ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
 
 
-def CholeskyAdjoint(L, Lbar, nb=64):
+def CholeskyAdjoint(L, Lbar, nb=0):
     """Reverse-mode derivative of DecomposeCHOL(A, LOWER, nb) (no counterpart
     in the reference). L: lower Cholesky factor; Lbar: d loss / dL (lower
     triangle read). Lbar is overwritten with the symmetric d loss / dA and

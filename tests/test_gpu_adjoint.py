@@ -1,9 +1,10 @@
 """GPU parity of the Cholesky adjoint (mops_dpotrf_adjoint /
-mops_d_dpotrf_adjoint; the GPU evaluates the closed form with an explicit
-L^-1 and triangle-aware products, the oracle with two triangular solves):
-against the oracle, the A = I special case, central finite differences through the
-Cholesky (h = 1e-6, 20 symmetric directions, unit Frobenius norm), torch's
-own Cholesky backward, and at the full n = 4096 workload
+mops_d_dpotrf_adjoint; the GPU evaluates the closed form in place by a
+blocked TRMM and two blocked TRSMs, the oracle with its own GEMM and two
+triangular solves): against the oracle (block orders 64 to 1024, with
+partial last blocks), the A = I special case, central finite differences
+through the Cholesky (h = 1e-6, 20 symmetric directions, unit Frobenius
+norm), torch's own Cholesky backward, and at the full n = 4096 workload
 (A = Q diag(logspace(0, 4)) Q^T, W lower N(0,1)) by finite differences.
 """
 import ctypes
@@ -36,7 +37,8 @@ def _sym_dirs(n, count, seed):
         yield E / np.linalg.norm(E)
 
 
-@pytest.mark.parametrize("n,nb", [(1, 0), (5, 0), (64, 64), (100, 32), (200, 64), (333, 0)])
+@pytest.mark.parametrize("n,nb", [(1, 0), (5, 0), (64, 64), (100, 32), (200, 64), (333, 0),
+                                  (700, 256), (1030, 0), (1100, 1024)])
 def test_adjoint_vs_oracle_closed_form(n, nb):
     A = _spd(n, 40 + n)
     L = np.linalg.cholesky(A)
