@@ -9,6 +9,7 @@
  * uses two alternating partial sums, like the 2-lane SSE accumulators of
  * calgo/inner_ddot.h:13.
  */
+#include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
 
@@ -493,9 +494,41 @@ static void p_dmmat_trmupd_blk(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t
   }
 }
 
+/* dvec_iamax, calgo/dvec_util.c:164-195: elements are taken in pairs; the
+ * larger of a pair (the first on a tie) becomes the current maximum if it
+ * is strictly above it (starting from 0.0); an odd last element likewise.
+ * Index 0 when nothing is above 0.0 (all zeros, NaN). A NaN first element
+ * of a pair hides its partner: the pair's candidate is then NaN, which
+ * never compares above the maximum. */
+static int p_dvec_iamax(const o_mvec_t *X, int N) {
+  if (N <= 1) return 0;
+  const double *Xc = X->md;
+  double max = 0.0;
+  int ix = 0, i;
+  for (i = 0; i < N - 1; i += 2) {
+    double c0 = fabs(Xc[0]);
+    const double c1 = fabs(Xc[X->inc]);
+    int n = 0;
+    if (c1 > c0) {
+      n = 1;
+      c0 = c1;
+    }
+    if (c0 > max) {
+      ix = i + n;
+      max = c0;
+    }
+    Xc += 2 * (ptrdiff_t)X->inc;
+  }
+  if (i < N) {
+    const double c0 = fabs(Xc[0]);
+    ix = c0 > max ? N - 1 : ix;
+  }
+  return ix;
+}
+
 const okern_t o_port_kernels = {
     p_dmult_mm_blocked3, p_dmmat_solve_blk,    p_dmmat_rank_blk, p_dmvec_symv_rank,
     p_dmvec_solve_blocked, p_dmult_gemv_blocked, p_dvec_dot,       p_dvec_swap,
     p_dvec_invscal,      p_dmmat_solve_unb,    p_dmult_symm_blocked2, p_dmmat_trmm_blk,
-    p_dmmat_rank2_blk,   p_dmmat_trmupd_blk,
+    p_dmmat_rank2_blk,   p_dmmat_trmupd_blk,   p_dvec_iamax,
 };

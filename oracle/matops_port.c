@@ -58,6 +58,7 @@ int o_use_kernels(int which, const char *path) {
   SYM(dmmat_trmm_blk)
   SYM(dmmat_rank2_blk)
   SYM(dmmat_trmupd_blk)
+  SYM(dvec_iamax)
 #undef SYM
   K = k;
   K_init = 1;
@@ -72,7 +73,7 @@ int o_kernels_in_use(void) { return K_which; }
  * dmmat_rank_blk, 3 dmvec_symv_rank, 4 dmvec_solve_blocked, 5
  * dmult_gemv_blocked, 6 dvec_dot, 7 dvec_swap, 8 dvec_invscal, 9
  * dmmat_solve_unb, 10 dmult_symm_blocked2, 11 dmmat_trmm_blk, 12
- * dmmat_rank2_blk, 13 dmmat_trmupd_blk). Used to
+ * dmmat_rank2_blk, 13 dmmat_trmupd_blk, 14 dvec_iamax). Used to
  * localise differences between the restatement and the reference. */
 int o_use_kernel_mask(int mask, const char *path) {
   if (o_use_kernels(1, path) != 0) return -1;
@@ -851,6 +852,499 @@ int o_InverseTrm(oview A, int flags, int nb) { /* trinv.go:219-245 */
   }
   return 0;
 }
+
+/* ---- DecomposeLDL / SolveLDL, ldl.go (diagonal pivoting) ----
+ * FloatMatrix helpers the file uses, restated: Diag (a 1 x n view stepping
+ * by ld + 1), CopyTo on vectors (element k to element k, for the shorter
+ * of the two), MVUpdateDiag (diag.go:141-166), SolveDiag / MultDiag with a
+ * diagonal taken from a square block (diag.go:14-139). */
+static oview diagv(oview A) {
+  int n = A.rows < A.cols ? A.rows : A.cols;
+  oview v = {A.p, 1, n, A.ld + 1};
+  if (n == 1) v.cols = 1;
+  return v;
+}
+static double *vref(oview X, int k) {
+  return X.rows == 1 ? &X.p[(ptrdiff_t)k * X.ld] : &X.p[k];
+}
+static void copyToV(oview src, oview dst) {
+  int n = numel(src) < numel(dst) ? numel(src) : numel(dst);
+  for (int k = 0; k < n; k++) *vref(dst, k) = *vref(src, k);
+}
+static void copyTo(oview src, oview dst) { /* same shape */
+  for (int j = 0; j < src.cols; j++)
+    for (int i = 0; i < src.rows; i++) dst.p[i + (ptrdiff_t)j * dst.ld] = src.p[i + (ptrdiff_t)j * src.ld];
+}
+static int IAMax(oview X) { /* mvec.go:405-420 -> DIAMax (cmops.go:619-634) */
+  if (X.rows != 1 && X.cols != 1) return -1;
+  int N = numel(X);
+  if (N <= 0) return -1;
+  o_mvec_t v = mv_cols(X);
+  return kern()->dvec_iamax(&v, N);
+}
+int o_iamax(const double *x, int inc, int n) {
+  if (n <= 0) return -1;
+  o_mvec_t v = {(double *)x, inc};
+  return kern()->dvec_iamax(&v, n);
+}
+static void MVUpdateDiag(oview d, oview x, oview y, double alpha) {
+  for (int k = 0; k < numel(d); k++) *vref(d, k) += velem(x, k) * velem(y, k) * alpha;
+}
+static void swapCols(oview A, int src, int dst) { /* pivot.go:37-45 */
+  if (src == dst || A.rows == 0) return;
+  Swap(sub(A, 0, src, A.rows, 1), sub(A, 0, dst, A.rows, 1));
+}
+static void scalePivots(int64_t *p, int n, int offset) { /* pivot.go:47-53 */
+  for (int k = 0; k < n; k++)
+    if (p[k] > 0) p[k] += offset;
+}
+/* applyRowPivots / applyColPivots, pivot.go:63-97 (dir 1 FORWARD, 2 BACKWARD) */
+static void applyRowPivotsD(oview A, const int64_t *p, int n, int offset, int dir) {
+  if (dir == 1) {
+    for (int k = 0; k < n; k++)
+      if (p[k] > 0) swapRows(A, (int)p[k] - 1 - offset, k);
+  } else {
+    for (int k = n - 1; k >= 0; k--)
+      if (p[k] > 0) swapRows(A, (int)p[k] - 1 - offset, k);
+  }
+}
+static void applyColPivotsD(oview A, const int64_t *p, int n, int offset, int dir) {
+  if (dir == 1) {
+    for (int k = 0; k < n; k++)
+      if (p[k] > 0) swapCols(A, (int)p[k] - 1 - offset, k);
+  } else {
+    for (int k = n - 1; k >= 0; k--)
+      if (p[k] > 0) swapCols(A, (int)p[k] - 1 - offset, k);
+  }
+}
+
+/* applyPivotSym, pivot.go:145-228: diagonal interchange of AR[0, 0] with
+ * AR[index, index] in a symmetric matrix stored in one triangle. LOWER: AL
+ * is the block left of AR (its rows swap too); UPPER: AL is [ATL ATR] and AR
+ * the row [a11 a12]. */
+static void applyPivotSym(oview AL, oview AR, int index, int flags) {
+  if (flags & O_LOWER) {
+    Swap(sub(AL, 0, 0, 1, AL.cols), sub(AL, index, 0, 1, AL.cols));
+    Swap(sub(AR, 1, 0, index - 1, 1), sub(AR, index, 1, 1, index - 1));
+    Swap(sub(AR, index + 1, 0, AR.rows - index - 1, 1),
+         sub(AR, index + 1, index, AR.rows - index - 1, 1));
+    double p1 = AR.p[0], p3 = AR.p[index + (ptrdiff_t)index * AR.ld];
+    AR.p[0] = p3;
+    AR.p[index + (ptrdiff_t)index * AR.ld] = p1;
+    return;
+  }
+  if (flags & O_UPPER) {
+    int colno = AL.cols - AR.cols;
+    Swap(sub(AR, 0, 1, 1, AR.cols - 1), sub(AL, index, colno + 1, 1, AR.cols - 1));
+    Swap(sub(AL, index + 1, colno, AL.rows - index - 2, 1),
+         sub(AL, index, index + 1, 1, colno - index - 1));
+    Swap(sub(AL, 0, index, index, 1), sub(AL, 0, colno, index, 1));
+    double p1 = AR.p[0], p3 = AL.p[index + (ptrdiff_t)index * AL.ld];
+    AR.p[0] = p3;
+    AL.p[index + (ptrdiff_t)index * AL.ld] = p1;
+  }
+}
+
+static int unblkLowerLDL(oview A, int64_t *p, const char **msg) { /* ldl.go:25-88 */
+  for (int k = 0; k < A.rows; k++) {
+    oview ABL = sub(A, k, 0, A.rows - k, k), ABR = sub(A, k, k, A.rows - k, A.cols - k);
+    int imax = IAMax(diagv(ABR));
+    if (imax > 0) {
+      applyPivotSym(ABL, ABR, imax, O_LOWER);
+      p[k] = imax + k + 1;
+    } else {
+      p[k] = 0;
+    }
+    double a11 = ABR.p[0];
+    if (a11 == 0.0) {
+      *msg = "zero value on diagonal";
+      return 1;
+    }
+    oview a21 = sub(A, k + 1, k, A.rows - k - 1, 1);
+    MVUpdateTrm(sub(A, k + 1, k + 1, A.rows - k - 1, A.cols - k - 1), a21, a21, -1.0 / a11, O_LOWER);
+    InvScale(a21, a11);
+  }
+  return 0;
+}
+
+/* findAndBuildPivot, ldl.go:90-133. The reference's column views of WR run
+ * over all rows of W; the rows past AR's are computed but never read back,
+ * so they are left out here (the reference's GEMV reads past the end of AL
+ * for them). */
+static int findAndBuildPivot(oview AL, oview AR, oview WL, oview WR, int k) {
+  oview dg = sub(WR, 0, WR.cols - 1, AR.rows, 1);
+  int dmax = IAMax(dg);
+  oview wcol = sub(WR, 0, 0, AR.rows, 1);
+  if (dmax == 0) {
+    copyToV(sub(AR, 0, 0, AR.rows, 1), wcol);
+  } else {
+    copyToV(sub(AR, dmax, 0, 1, dmax + 1), wcol);
+    if (dmax < AR.rows - 1)
+      copyToV(sub(AR, dmax, dmax, AR.rows - dmax, 1), sub(WR, dmax, 0, wcol.rows - dmax, 1));
+  }
+  if (k > 0) MVMult(wcol, AL, sub(WL, dmax, 0, 1, WL.cols), -1.0, 1.0, 0);
+  if (dmax > 0) {
+    double t0 = WR.p[0];
+    WR.p[0] = WR.p[dmax];
+    WR.p[dmax] = t0;
+    /* dg is WR's last column: the same column as wcol when WR has one */
+    t0 = dg.p[0];
+    dg.p[0] = dg.p[dmax];
+    dg.p[dmax] = t0;
+  }
+  return dmax;
+}
+
+static int unblkBoundedLowerLDL(oview A, oview W, int64_t *p, int ncol) { /* ldl.go:135-216 */
+  copyToV(diagv(A), sub(W, 0, W.cols - 1, A.rows, 1));
+  int nc = 0;
+  while (A.cols - nc > 0 && nc < ncol) {
+    oview w10 = sub(W, nc, 0, W.rows - nc, nc), w11 = sub(W, nc, nc, W.rows - nc, W.cols - nc);
+    oview ABL = sub(A, nc, 0, A.rows - nc, nc), ABR = sub(A, nc, nc, A.rows - nc, A.cols - nc);
+    int dmax = findAndBuildPivot(ABL, ABR, w10, w11, nc);
+    if (dmax > 0) {
+      applyPivotSym(ABL, ABR, dmax, O_LOWER);
+      swapRows(w10, 0, dmax);
+      p[nc] = dmax + nc + 1;
+    } else {
+      p[nc] = 0;
+    }
+    oview a21 = sub(A, nc + 1, nc, A.rows - nc - 1, 1);
+    oview wc = sub(w11, 1, 0, a21.rows, 1);
+    copyToV(wc, a21);
+    A.p[nc + (ptrdiff_t)nc * A.ld] = w11.p[0];
+    InvScale(a21, A.p[nc + (ptrdiff_t)nc * A.ld]);
+    if (ncol - nc > 1)
+      MVUpdateDiag(sub(w11, 1, w11.cols - 1, a21.rows, 1), wc, wc,
+                   -1.0 / A.p[nc + (ptrdiff_t)nc * A.ld]);
+    nc++;
+  }
+  return nc;
+}
+
+static int blkLowerLDL(oview A, oview W, int64_t *p, int nb, const char **msg) { /* ldl.go:227-295 */
+  int k = 0;
+  while (A.cols - k > nb) {
+    int nblk = unblkBoundedLowerLDL(sub(A, k, k, A.rows - k, A.cols - k), W, p + k, nb);
+    oview A21 = sub(A, k + nblk, k, A.rows - k - nblk, nblk);
+    oview A22 = sub(A, k + nblk, k + nblk, A.rows - k - nblk, A.cols - k - nblk);
+    o_UpdateTrm(A22, A21, sub(W, nblk, 0, A21.rows, nblk), -1.0, 1.0, O_LOWER | O_TRANSB);
+    applyRowPivotsD(sub(A, k, 0, A.rows - k, k), p + k, nblk, 0, 1);
+    scalePivots(p + k, nblk, k);
+    k += nblk;
+  }
+  if (A.cols - k > 0) {
+    oview ABR = sub(A, k, k, A.rows - k, A.cols - k);
+    if (unblkLowerLDL(ABR, p + k, msg)) return 1;
+    applyRowPivotsD(sub(A, k, 0, A.rows - k, k), p + k, ABR.rows, 0, 1);
+    scalePivots(p + k, ABR.rows, k);
+  }
+  return 0;
+}
+
+static int unblkUpperLDL(oview A, int64_t *p, const char **msg) { /* ldl.go:297-352 */
+  for (int k = A.rows - 1; k >= 0; k--) {
+    int imax = IAMax(diagv(sub(A, 0, 0, k + 1, k + 1)));
+    if (imax < k) {
+      applyPivotSym(sub(A, 0, 0, k + 1, A.cols), sub(A, k, k, 1, A.cols - k), imax, O_UPPER);
+      p[k] = imax + 1;
+    } else {
+      p[k] = 0;
+    }
+    double a11 = A.p[k + (ptrdiff_t)k * A.ld];
+    if (a11 == 0.0) {
+      *msg = "zero on diagonal.";
+      return 1;
+    }
+    oview a01 = sub(A, 0, k, k, 1);
+    MVUpdateTrm(sub(A, 0, 0, k, k), a01, a01, -1.0 / a11, O_UPPER);
+    InvScale(a01, a11);
+  }
+  return 0;
+}
+
+static int blkUpperLDL(oview A, oview W, int64_t *p, int nb, const char **msg) { /* ldl.go:356-418 */
+  for (int e = A.rows; e > 0;) {
+    int kb = e < nb ? e : nb, k = e - kb;
+    oview A01 = sub(A, 0, k, k, kb), A11 = sub(A, k, k, kb, kb);
+    oview A12 = sub(A, k, k + kb, kb, A.cols - k - kb);
+    if (unblkUpperLDL(A11, p + k, msg)) return 1;
+    applyColPivotsD(A01, p + k, kb, 0, 2);
+    applyRowPivotsD(A12, p + k, kb, 0, 2);
+    scalePivots(p + k, kb, e - kb);
+    o_SolveTrm(A01, A11, 1.0, O_UPPER | O_UNIT | O_RIGHT | O_TRANSA);
+    solveDiagRight(A01, A11);
+    if (k > 0) {
+      /* W = U01 D1 (MultDiag RIGHT) */
+      oview wrk = sub(W, 0, 0, k, kb);
+      copyTo(A01, wrk);
+      for (int j = 0; j < kb; j++) Scale(sub(wrk, 0, j, k, 1), A11.p[(ptrdiff_t)j * (A11.ld + 1)]);
+      o_UpdateTrm(sub(A, 0, 0, k, k), A01, wrk, -1.0, 1.0, O_UPPER | O_TRANSB);
+    }
+    e = k;
+  }
+  return 0;
+}
+
+int o_DecomposeLDL(oview A, oview W, int64_t *ipiv, int npiv, int flags, int nb,
+                   const char **msg) { /* ldl.go:439-484 */
+  *msg = 0;
+  if (A.cols != A.rows) {
+    *msg = "A not a square matrix";
+    return 1;
+  }
+  for (int k = 0; k < npiv; k++) ipiv[k] = 0;
+  if (A.cols < nb || nb == 0)
+    return flags & O_LOWER ? unblkLowerLDL(A, ipiv, msg) : unblkUpperLDL(A, ipiv, msg);
+  return flags & O_LOWER ? blkLowerLDL(A, W, ipiv, nb, msg) : blkUpperLDL(A, W, ipiv, nb, msg);
+}
+
+/* SolveDiag(B, A, LEFT) with A square (diag.go:85-139): rows times 1/d_k;
+ * a 1 x 1 A is a column vector there, and the column is divided. */
+static void solveDiagLeft(oview B, oview A) {
+  if (A.cols == 1) {
+    for (int k = 0; k < B.cols; k++) B.p[(ptrdiff_t)k * B.ld] /= A.p[0];
+    return;
+  }
+  for (int k = 0; k < B.rows; k++) Scale(sub(B, k, 0, 1, B.cols), 1.0 / A.p[(ptrdiff_t)k * (A.ld + 1)]);
+}
+
+void o_SolveLDL(oview B, oview A, const int64_t *ipiv, int npiv, int flags) { /* ldl.go:486-511 */
+  if (flags & O_UPPER) {
+    applyRowPivotsD(B, ipiv, npiv, 0, 2);
+    o_SolveTrm(B, A, 1.0, O_UPPER | O_UNIT);
+    solveDiagLeft(B, A);
+    o_SolveTrm(B, A, 1.0, O_UPPER | O_UNIT | O_TRANSA);
+    applyRowPivotsD(B, ipiv, npiv, 0, 1);
+  } else if (flags & O_LOWER) {
+    applyRowPivotsD(B, ipiv, npiv, 0, 1);
+    o_SolveTrm(B, A, 1.0, O_LOWER | O_UNIT);
+    solveDiagLeft(B, A);
+    o_SolveTrm(B, A, 1.0, O_LOWER | O_UNIT | O_TRANSA);
+    applyRowPivotsD(B, ipiv, npiv, 0, 2);
+  }
+}
+
+/* ---- DecomposeBK / SolveBK, ldlbk.go, ldlbks.go (Bunch-Kaufman) ----
+ * Restated: the unblocked factorizations (ldlbk.go:212-456, the forms of
+ * LAPACK's dsytf2) and the LOWER solve. DecomposeBK's blocked branch (the
+ * dlasyf-like ldlbk.go:459-1111) is not restated: the oracle runs the
+ * unblocked algorithm for it, which makes the same pivot decisions on the
+ * same (exactly updated) values. SolveBK's UPPER branch (ldlbks.go:159-301)
+ * is not restated: it steps its pivot counter the wrong way and indexes
+ * past the pivot array at its second step. */
+static const double bkALPHA = 0.6403882032022075; /* (1 + sqrt(17)) / 8 */
+
+static void applyBKPivotSym(oview AR, int srcix, int dstix, int flags) { /* ldlbk.go:48-105 */
+  if (flags & O_LOWER) {
+    Swap(sub(AR, srcix + 1, srcix, dstix - srcix - 1, 1), sub(AR, dstix, srcix + 1, 1, dstix - srcix - 1));
+    Swap(sub(AR, dstix + 1, srcix, AR.rows - dstix - 1, 1), sub(AR, dstix + 1, dstix, AR.rows - dstix - 1, 1));
+  } else if (flags & O_UPPER) {
+    Swap(sub(AR, dstix + 1, srcix, srcix - dstix - 1, 1), sub(AR, dstix, dstix + 1, 1, srcix - dstix - 1));
+    Swap(sub(AR, 0, srcix, dstix, 1), sub(AR, 0, dstix, dstix, 1));
+  } else {
+    return;
+  }
+  double p1 = AR.p[srcix + (ptrdiff_t)srcix * AR.ld], p3 = AR.p[dstix + (ptrdiff_t)dstix * AR.ld];
+  AR.p[srcix + (ptrdiff_t)srcix * AR.ld] = p3;
+  AR.p[dstix + (ptrdiff_t)dstix * AR.ld] = p1;
+}
+
+#define AT(M, i, j) ((M).p[(i) + (ptrdiff_t)(j) * (M).ld])
+
+static void findBKPivot(oview A, int flags, int *r_out, int *np_out) { /* ldlbk.go:107-210 */
+  *r_out = 0;
+  *np_out = 1;
+  if (A.rows == 1) return;
+  if (flags & O_LOWER) {
+    double amax = fabs(AT(A, 0, 0));
+    int r = IAMax(sub(A, 1, 0, A.rows - 1, 1)) + 1;
+    double rmax = fabs(AT(A, r, 0));
+    if (amax >= bkALPHA * rmax) return;
+    int q = IAMax(sub(A, r, 0, 1, r));
+    double qmax = fabs(AT(A, r, q));
+    if (r < A.rows - 1) {
+      oview qrow = sub(A, r + 1, r, A.rows - r - 1, 1);
+      q = IAMax(qrow);
+      double qmax2 = fabs(AT(qrow, q, 0));
+      if (qmax2 > qmax) qmax = qmax2;
+    }
+    if (amax >= bkALPHA * rmax * (rmax / qmax)) return;
+    *r_out = r;
+    *np_out = fabs(AT(A, r, r)) >= bkALPHA * qmax ? 1 : 2;
+    return;
+  }
+  if (flags & O_UPPER) {
+    int lastcol = A.rows - 1;
+    double amax = fabs(AT(A, lastcol, lastcol));
+    int r = IAMax(sub(A, 0, lastcol, lastcol, 1));
+    double rmax = fabs(AT(A, r, lastcol));
+    *r_out = -1;
+    if (amax >= bkALPHA * rmax) return;
+    double qmax = 0.0;
+    if (r > 0) {
+      int q = IAMax(sub(A, 0, r, r, 1));
+      qmax = fabs(AT(A, q, r));
+    }
+    oview qrow = sub(A, r, r + 1, 1, lastcol - r);
+    int q = IAMax(qrow);
+    double qmax2 = fabs(AT(qrow, 0, q));
+    if (qmax2 > qmax) qmax = qmax2;
+    if (amax >= bkALPHA * rmax * (rmax / qmax)) return;
+    *r_out = r;
+    *np_out = fabs(AT(A, r, r)) >= bkALPHA * qmax ? 1 : 2;
+  }
+}
+
+/* the inverse of the 2 x 2 pivot block [a b; b d], as ldlbk.go:274-285:
+ * inv = scale [d/b -1; -1 a/b], scale = 1 / ((a/b)(d/b) - 1) / b */
+static double bk_inverse(oview a11inv, double a, double b, double d) {
+  AT(a11inv, 0, 0) = d / b;
+  AT(a11inv, 1, 1) = a / b;
+  double scale = 1.0 / ((a / b) * (d / b) - 1.0);
+  return scale / b;
+}
+
+static void unblkDecompBKLower(oview A, oview wrk, int64_t *p) { /* ldlbk.go:212-332 */
+  oview a11inv = sub(wrk, 0, wrk.cols - 2, 2, 2);
+  AT(a11inv, 1, 0) = -1.0;
+  AT(a11inv, 0, 1) = -1.0;
+  for (int k = 0; A.cols - k > 0;) {
+    oview ABR = sub(A, k, k, A.rows - k, A.cols - k);
+    int r, np;
+    findBKPivot(ABR, O_LOWER, &r, &np);
+    if (r != 0 && r != np - 1) {
+      applyBKPivotSym(ABR, np - 1, r, O_LOWER);
+      if (np == 2) {
+        double t = AT(ABR, 1, 0);
+        AT(ABR, 1, 0) = AT(ABR, r, 0);
+        AT(ABR, r, 0) = t;
+      }
+    }
+    oview a11 = sub(A, k, k, np, np), a21 = sub(A, k + np, k, A.rows - k - np, np);
+    oview A22 = sub(A, k + np, k + np, A.rows - k - np, A.cols - k - np);
+    if (np == 1) {
+      MVUpdateTrm(A22, a21, a21, -1.0 / AT(a11, 0, 0), O_LOWER);
+      InvScale(a21, AT(a11, 0, 0));
+      p[k] = r + k + 1;
+    } else {
+      double scale = bk_inverse(a11inv, AT(a11, 0, 0), AT(a11, 1, 0), AT(a11, 1, 1));
+      oview cwrk = sub(wrk, 2, 0, a21.rows, a21.cols);
+      copyTo(a21, cwrk);
+      o_Mult(a21, cwrk, a11inv, scale, 0.0, 0);
+      o_UpdateTrm(A22, a21, cwrk, -1.0, 1.0, O_LOWER | O_TRANSB);
+      p[k] = -(r + k + 1);
+      p[k + 1] = p[k];
+    }
+    k += np;
+  }
+}
+
+static void unblkDecompBKUpper(oview A, oview wrk, int64_t *p) { /* ldlbk.go:335-456 */
+  oview a11inv = sub(wrk, 0, wrk.cols - 2, 2, 2);
+  AT(a11inv, 1, 0) = -1.0;
+  AT(a11inv, 0, 1) = -1.0;
+  for (int e = A.rows; e > 0;) {
+    oview ATL = sub(A, 0, 0, e, e);
+    int nr = e - 1, r, np;
+    findBKPivot(ATL, O_UPPER, &r, &np);
+    if (r != -1) {
+      applyBKPivotSym(ATL, e - np, r, O_UPPER);
+      if (np == 2) {
+        double t = AT(ATL, nr - 1, nr);
+        AT(ATL, nr - 1, nr) = AT(ATL, r, nr);
+        AT(ATL, r, nr) = t;
+      }
+    }
+    int k = e - np;
+    oview a11 = sub(A, k, k, np, np), a01 = sub(A, 0, k, k, np), A00 = sub(A, 0, 0, k, k);
+    if (np == 1) {
+      MVUpdateTrm(A00, a01, a01, -1.0 / AT(a11, 0, 0), O_UPPER);
+      InvScale(a01, AT(a11, 0, 0));
+      p[k] = r == -1 ? e : r + 1;
+    } else {
+      double scale = bk_inverse(a11inv, AT(a11, 0, 0), AT(a11, 0, 1), AT(a11, 1, 1));
+      oview cwrk = sub(wrk, 2, 0, a01.rows, a01.cols);
+      copyTo(a01, cwrk);
+      o_Mult(a01, cwrk, a11inv, scale, 0.0, 0);
+      o_UpdateTrm(A00, a01, cwrk, -1.0, 1.0, O_UPPER | O_TRANSB);
+      p[k] = -(r + 1);
+      p[k + 1] = p[k];
+    }
+    e -= np;
+  }
+}
+
+int o_DecomposeBK(oview A, oview W, int64_t *ipiv, int npiv, int flags, int nb,
+                  const char **msg) { /* ldlbk.go:1114-1140 */
+  *msg = 0;
+  for (int k = 0; k < npiv; k++) ipiv[k] = 0;
+  const int blocked = !(A.cols < nb || nb == 0);
+  if (blocked ? (W.cols < nb + 1 || W.rows < A.rows) : (W.cols < 2 || W.rows < A.rows)) {
+    *msg = "Workspace too small";
+    return 1;
+  }
+  if (flags & O_LOWER)
+    unblkDecompBKLower(A, W, ipiv);
+  else if (flags & O_UPPER)
+    unblkDecompBKUpper(A, W, ipiv);
+  return 0;
+}
+
+/* unblkSolveBKLower, ldlbks.go:15-157: phase 1 Z = D^-1 L^-1 P B with the
+ * interchanges applied step by step, phase 2 X = P^T L^-T Z */
+static void solveBKLower(oview B, oview A, const int64_t *p, int phase) {
+  const int n = A.rows;
+  if (phase == 1) {
+    for (int k = 0; k < n;) { /* k = BT.Rows(), nc = k + 1 */
+      int np = p[k] < 0 ? 2 : 1;
+      oview BB = sub(B, k, 0, B.rows - k, B.cols);
+      oview b1 = sub(B, k, 0, np, B.cols), B2 = sub(B, k + np, 0, B.rows - k - np, B.cols);
+      oview a11 = sub(A, k, k, np, np), a21 = sub(A, k + np, k, n - k - np, np);
+      if (np == 1) {
+        if (p[k] != k + 1) swapRows(BB, 0, (int)p[k] - k - 1);
+        MVRankUpdate(B2, a21, b1, -1.0);
+        InvScale(b1, AT(a11, 0, 0));
+      } else {
+        if (p[k] != -(k + 1)) swapRows(BB, 1, (int)(-p[k]) - k - 1);
+        double b = AT(a11, 1, 0), apb = AT(a11, 0, 0) / b, dpb = AT(a11, 1, 1) / b;
+        double scale = (apb * dpb - 1.0) * b;
+        o_Mult(B2, a21, b1, -1.0, 1.0, 0);
+        for (int j = 0; j < b1.cols; j++) {
+          double s0 = AT(b1, 0, j), s1 = AT(b1, 1, j);
+          AT(b1, 0, j) = (dpb * s0 - s1) / scale;
+          AT(b1, 1, j) = (apb * s1 - s0) / scale;
+        }
+      }
+      k += np;
+    }
+    return;
+  }
+  for (int nc = n; nc > 0;) { /* nc: 1-based row of the block's last row */
+    int np = p[nc - 1] < 0 ? 2 : 1;
+    int k = nc - np; /* first row of the block; BT.Rows() = nc */
+    oview b1 = sub(B, k, 0, np, B.cols), B2 = sub(B, nc, 0, B.rows - nc, B.cols);
+    oview a21 = sub(A, nc, k, n - nc, np);
+    oview Bx = sub(B, k, 0, B.rows - k, B.cols);
+    if (np == 1) {
+      MVMult(b1, B2, a21, -1.0, 1.0, O_TRANSA);
+      if (p[k] != nc) swapRows(Bx, 0, (int)p[k] - nc);
+    } else {
+      o_Mult(b1, a21, B2, -1.0, 1.0, O_TRANSA);
+      if (p[k] != -nc) swapRows(Bx, 1, (int)(-p[k]) - nc + 1);
+    }
+    nc -= np;
+  }
+}
+
+void o_SolveBK(oview B, oview A, const int64_t *ipiv, int npiv, int flags) { /* ldlbks.go:303-316 */
+  (void)npiv;
+  if (flags & O_LOWER) {
+    solveBKLower(B, A, ipiv, 1);
+    solveBKLower(B, A, ipiv, 2);
+  }
+}
+#undef AT
 
 /* ---- reference Cholesky adjoint (no counterpart in matops) ----
  * Abar = 1/2 (S + S^T), S = L^-T Phi(L^T Lbar) L^-1 (the closed form of the

@@ -58,6 +58,7 @@ typedef struct {
                           double beta, int flags, int P, int S, int E, int vlen, int NB);
   void (*dmmat_trmupd_blk)(o_mdata_t *C, const o_mdata_t *A, const o_mdata_t *B, double alpha,
                            double beta, int flags, int P, int S, int E, int vlen, int NB);
+  int (*dvec_iamax)(const o_mvec_t *X, int N);
 } okern_t;
 
 /* calgo_port.c: the restated kernels */
@@ -90,6 +91,16 @@ int o_DecomposeLUnoPiv(oview A, int nb);
 int o_DecomposeLDLnoPiv(oview A, int flags, int nb);
 void o_SolveLDLnoPiv(oview B, oview A, int flags);
 int o_InverseTrm(oview A, int flags, int nb);
+/* DecomposeLDL / SolveLDL (ldl.go:25-511) and DecomposeBK / SolveBK
+ * (ldlbk.go:48-458, 1114-1140; ldlbks.go). Pivots are Go ints (int64).
+ * Returns 0, 1 for an error with its text in *msg ("A not a square matrix",
+ * "zero value on diagonal", "zero on diagonal.", "Workspace too small"). */
+int o_DecomposeLDL(oview A, oview W, int64_t *ipiv, int npiv, int flags, int nb, const char **msg);
+void o_SolveLDL(oview B, oview A, const int64_t *ipiv, int npiv, int flags);
+int o_DecomposeBK(oview A, oview W, int64_t *ipiv, int npiv, int flags, int nb, const char **msg);
+void o_SolveBK(oview B, oview A, const int64_t *ipiv, int npiv, int flags);
+/* IAMax of the reference (mvec.go:405-420 -> dvec_iamax) on n elements */
+int o_iamax(const double *x, int inc, int n);
 
 /* Reference adjoint of the Cholesky factorization (closed form, built on
  * the oracle's TRSM/GEMM): given the lower factor L and Lbar (lower),

@@ -67,6 +67,11 @@ def lib():
             "o_DecomposeLDLnoPiv": (I, [V, I, I]),
             "o_SolveLDLnoPiv": (None, [V, V, I]),
             "o_InverseTrm": (I, [V, I, I]),
+            "o_DecomposeLDL": (I, [V, V, P, I, I, I, P]),
+            "o_SolveLDL": (None, [V, V, P, I, I]),
+            "o_DecomposeBK": (I, [V, V, P, I, I, I, P]),
+            "o_SolveBK": (None, [V, V, P, I, I]),
+            "o_iamax": (I, [P, I, I]),
             "o_chol_adjoint": (None, [I, P, I, P, I, P, I]),
             "o_now": (D, []),
             "o_kern_gemm": (None, [P, I, P, I, P, I, D, D, I, I, I, I, I, I]),
@@ -161,6 +166,46 @@ def SolveLDLnoPiv(B, A, flags):
 
 def InverseTrm(A, flags, nb):
     return lib().o_InverseTrm(_v(A), flags, nb)
+
+
+def _w(W, n):
+    return W if W is not None else np.zeros((max(n, 1), 2), order="F")
+
+
+def DecomposeLDL(A, W, ipiv, flags, nb):
+    """ldl.go:439-484. Returns None or the error text."""
+    assert ipiv.dtype == np.int64 and ipiv.flags.c_contiguous
+    msg = ctypes.c_char_p()
+    rc = lib().o_DecomposeLDL(_v(A), _v(_w(W, A.shape[0])), ipiv.ctypes.data, len(ipiv), flags,
+                              nb, ctypes.addressof(msg))
+    return msg.value.decode() if rc else None
+
+
+def SolveLDL(B, A, ipiv, flags):
+    assert ipiv.dtype == np.int64
+    lib().o_SolveLDL(_v(B), _v(A), ipiv.ctypes.data, len(ipiv), flags)
+
+
+def DecomposeBK(A, W, ipiv, flags, nb):
+    """ldlbk.go:1114-1140 (the blocked branch runs the unblocked algorithm).
+    Returns None or the error text."""
+    assert ipiv.dtype == np.int64 and ipiv.flags.c_contiguous
+    msg = ctypes.c_char_p()
+    rc = lib().o_DecomposeBK(_v(A), _v(W), ipiv.ctypes.data, len(ipiv), flags, nb,
+                             ctypes.addressof(msg))
+    return msg.value.decode() if rc else None
+
+
+def SolveBK(B, A, ipiv, flags):
+    """ldlbks.go:303-316, LOWER only (the reference's UPPER branch fails)."""
+    assert ipiv.dtype == np.int64
+    lib().o_SolveBK(_v(B), _v(A), ipiv.ctypes.data, len(ipiv), flags)
+
+
+def iamax(x):
+    """IAMax of the selected kernels (dvec_iamax) on a contiguous vector."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().o_iamax(x.ctypes.data, 1, len(x))
 
 
 def DecomposeCHOL(A, flags, nb):
