@@ -129,6 +129,88 @@ func SolveLDLnoPiv(B, A *matrix.FloatMatrix, flags Flags) {
 	mustOK(C.mops_dsytrs_nopiv(&b, &a, C.int(flags), C.int(A.Rows()), C.int(B.Cols())))
 }
 
+// DecomposeLDL: A = P.T*L*D*L.T*P (LOWER) or P.T*U*D*U.T*P with 1x1 diagonal
+// pivots (ldl.go:439-460). W is not read (the GPU keeps its own workspace).
+// A zero pivot gives the reference's error (or panics under
+// SetPanicOnError).
+func DecomposeLDL(A, W *matrix.FloatMatrix, ipiv []int, flags Flags, nb int) (*matrix.FloatMatrix, error) {
+	if A.Cols() != A.Rows() {
+		return nil, errors.New("A not a square matrix")
+	}
+	for k := range ipiv {
+		ipiv[k] = 0
+	}
+	if A.Rows() == 0 {
+		return A, nil
+	}
+	if len(ipiv) < A.Rows() {
+		panic("runtime error: index out of range") // the reference's slice access
+	}
+	var pn runtime.Pinner
+	defer pn.Unpin()
+	a := view(&pn, A)
+	rc := C.mops_dsytrf_ldl(&a, pivotPtr(ipiv), C.int(len(ipiv)), C.int(flags), C.int(A.Rows()),
+		C.int(nb))
+	if rc > 0 {
+		return A, onError(C.GoString(C.mops_last_error()))
+	}
+	mustOK(rc)
+	return A, nil
+}
+
+// SolveLDL: B = A.-1*B from DecomposeLDL (ldl.go:486-511).
+func SolveLDL(B, A *matrix.FloatMatrix, ipiv []int, flags Flags) {
+	if A.Rows() == 0 || B.Cols() == 0 {
+		return
+	}
+	var pn runtime.Pinner
+	defer pn.Unpin()
+	b, a := view(&pn, B), view(&pn, A)
+	mustOK(C.mops_dsytrs_ldl(&b, &a, pivotPtr(ipiv), C.int(len(ipiv)), C.int(flags),
+		C.int(A.Rows()), C.int(B.Cols())))
+}
+
+// DecomposeBK: A = L*D*L.T (LOWER) or U*D*U.T (UPPER) with Bunch-Kaufman
+// pivoting (ldlbk.go:1114-1140); ipiv in LAPACK's format. W is checked as
+// the reference does and not read.
+func DecomposeBK(A, W *matrix.FloatMatrix, ipiv []int, flags Flags, nb int) (*matrix.FloatMatrix, error) {
+	for k := range ipiv {
+		ipiv[k] = 0
+	}
+	need := nb + 1
+	if A.Cols() < nb || nb == 0 {
+		need = 2
+	}
+	if W == nil || W.Cols() < need || W.Rows() < A.Rows() {
+		return nil, errors.New("Workspace too small")
+	}
+	if A.Rows() == 0 || flags&(LOWER|UPPER) == 0 {
+		return A, nil
+	}
+	if len(ipiv) < A.Rows() {
+		panic("runtime error: index out of range")
+	}
+	var pn runtime.Pinner
+	defer pn.Unpin()
+	a := view(&pn, A)
+	mustOK(C.mops_dsytrf_bk(&a, pivotPtr(ipiv), C.int(len(ipiv)), C.int(flags), C.int(A.Rows()),
+		C.int(nb)))
+	return A, nil
+}
+
+// SolveBK: B = A.-1*B from DecomposeBK (ldlbks.go:303-316); the UPPER
+// branch follows LAPACK's dsytrs.
+func SolveBK(B, A *matrix.FloatMatrix, ipiv []int, flags Flags) {
+	if A.Rows() == 0 || B.Cols() == 0 {
+		return
+	}
+	var pn runtime.Pinner
+	defer pn.Unpin()
+	b, a := view(&pn, B), view(&pn, A)
+	mustOK(C.mops_dsytrs_bk(&b, &a, pivotPtr(ipiv), C.int(len(ipiv)), C.int(flags),
+		C.int(A.Rows()), C.int(B.Cols())))
+}
+
 // InverseTrm: the LOWER (else upper) triangle of A replaced by that of its
 // inverse, UNIT for a unit diagonal (trinv.go:219-245).
 func InverseTrm(A *matrix.FloatMatrix, flags Flags, nb int) (*matrix.FloatMatrix, error) {

@@ -312,6 +312,64 @@ int mops_dsytrs_nopiv(mops_mdata_t *B, const mops_mdata_t *A, int flags, int N, 
 int mops_d_dsytrs_nopiv(int flags, int N, int nrhs, const double *A, int lda, double *B, int ldb,
                         void *stream);
 
+/* ---- symmetric indefinite factorizations with pivoting ------------------- */
+
+/* Replaces DecomposeLDL (ldl.go:439-460) and its blocked / unblocked forms
+ * (ldl.go:25-418): A (N x N) = P^T L D L^T P (LOWER) or P^T U D U^T P
+ * (otherwise) in place, 1x1 pivots (the largest |diagonal| of the part not
+ * yet factored). ipiv: npiv int64 entries (Go []int), all cleared, then for
+ * each k: k+1-based row exchanged with row k, 0 = none (ldl.go:427). The
+ * blocked UPPER form (nb > 0, N >= nb) pivots inside each diagonal block of
+ * order nb, as the reference does. The blocked LOWER form gives the
+ * unblocked result (the reference's does with a workspace of nb + 1
+ * columns; with nb columns its diagonal store collides with the panel).
+ * A zero pivot stops the factorization: returns 1 + its 0-based index with
+ * mops_last_error() "zero value on diagonal" (LOWER, ldl.go:66) or "zero on
+ * diagonal." (UPPER, ldl.go:335); in the blocked LOWER form only the
+ * columns the reference factors unblocked are checked (ldl.go:283). No
+ * workspace argument: the GPU keeps its own. Returns -3 when npiv < N.
+ * Host memory, synchronous. */
+int mops_dsytrf_ldl(mops_mdata_t *A, int64_t *ipiv, int npiv, int flags, int N, int nb);
+/* Device-resident twin: ipiv device int64[npiv]; info: device int (may be
+ * NULL), -1 on success, else the 0-based index of the zero pivot. The
+ * blocked UPPER form synchronizes the stream once per block. */
+int mops_d_dsytrf_ldl(int flags, int N, int nb, double *A, int lda, int64_t *ipiv, int npiv,
+                      int *info, void *stream);
+
+/* Replaces SolveLDL (ldl.go:486-511): B (N x nrhs) = A^-1 B from the factors
+ * and pivots of mops_dsytrf_ldl; UPPER selects U D U^T (checked first), else
+ * LOWER L D L^T; neither bit: B is left as is. Pivot entries outside
+ * [0, N] are ignored. Host memory, synchronous. */
+int mops_dsytrs_ldl(mops_mdata_t *B, const mops_mdata_t *A, const int64_t *ipiv, int npiv,
+                    int flags, int N, int nrhs);
+int mops_d_dsytrs_ldl(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,
+                      int npiv, double *B, int ldb, void *stream);
+
+/* Replaces DecomposeBK (ldlbk.go:1114-1140) with its unblocked and blocked
+ * forms (ldlbk.go:107-1111): A (N x N) = L D L^T (LOWER) or U D U^T (UPPER,
+ * else nothing but clearing ipiv), D with 1x1 and 2x2 blocks, Bunch-Kaufman
+ * pivoting (alpha = (1 + sqrt(17)) / 8). ipiv in LAPACK's format, 1-based:
+ * ipiv[k] > 0 a 1x1 block with rows k and ipiv[k]-1 exchanged (itself when
+ * none), ipiv[k] = ipiv[k+1] < 0 a 2x2 block. Every nb gives the unblocked
+ * decisions (the reference's blocked form makes the same). Singular blocks
+ * give Inf/NaN, not an error, as in the reference. The reference's
+ * "Workspace too small" check on its W argument stays in the caller.
+ * Returns -3 when npiv < N. Host memory, synchronous. */
+int mops_dsytrf_bk(mops_mdata_t *A, int64_t *ipiv, int npiv, int flags, int N, int nb);
+int mops_d_dsytrf_bk(int flags, int N, int nb, double *A, int lda, int64_t *ipiv, int npiv,
+                     void *stream);
+
+/* Replaces SolveBK (ldlbks.go:303-316): B (N x nrhs) = A^-1 B from the
+ * factors and pivots of mops_dsytrf_bk; LOWER (checked first) or UPPER,
+ * neither: B is left as is. The UPPER solve follows LAPACK's dsytrs (the
+ * reference's UPPER branch, ldlbks.go:159-301, steps its block counter the
+ * wrong way). Pivot entries outside [-N, N] are ignored. One workgroup per
+ * right-hand side. Host memory, synchronous. */
+int mops_dsytrs_bk(mops_mdata_t *B, const mops_mdata_t *A, const int64_t *ipiv, int npiv,
+                   int flags, int N, int nrhs);
+int mops_d_dsytrs_bk(int flags, int N, int nrhs, const double *A, int lda, const int64_t *ipiv,
+                     int npiv, double *B, int ldb, void *stream);
+
 /* Replaces InverseTrm (trinv.go:16-245): the LOWER (else upper) triangle of
  * A (N x N) replaced by that of its inverse; UNIT takes the diagonal as 1
  * and leaves it as stored; the other triangle is not written. nb is

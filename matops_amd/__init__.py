@@ -10,4 +10,5 @@ from .matops import (  # noqa: F401
     PREC_BF16, PREC_FP32, PREC_FP64, RankUpdateSym, SetPanicOnError, SolveCHOL, SolveIR, SolveLU, SolveTrm,
     blockIndex4, divideWork, scheduleWork, MultSym, MultTrm, RankUpdate2Sym, UpdateTrm,
     DecomposeLUnoPiv, DecomposeLDLnoPiv, SolveLDLnoPiv, InverseTrm,
+    DecomposeLDL, SolveLDL, DecomposeBK, SolveBK,
 )

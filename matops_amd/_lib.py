@@ -61,6 +61,14 @@ SIGNATURES = {
     "mops_d_dsytrf_nopiv": (_I, [_I, _I, _I, _P, _I, _P]),
     "mops_dsytrs_nopiv": (_I, [_PM, _PM, _I, _I, _I]),
     "mops_d_dsytrs_nopiv": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+    "mops_dsytrf_ldl": (_I, [_PM, _P, _I, _I, _I, _I]),
+    "mops_d_dsytrf_ldl": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _P]),
+    "mops_dsytrs_ldl": (_I, [_PM, _PM, _P, _I, _I, _I, _I]),
+    "mops_d_dsytrs_ldl": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _I, _P]),
+    "mops_dsytrf_bk": (_I, [_PM, _P, _I, _I, _I, _I]),
+    "mops_d_dsytrf_bk": (_I, [_I, _I, _I, _P, _I, _P, _I, _P]),
+    "mops_dsytrs_bk": (_I, [_PM, _PM, _P, _I, _I, _I, _I]),
+    "mops_d_dsytrs_bk": (_I, [_I, _I, _I, _P, _I, _P, _I, _P, _I, _P]),
     "mops_dinverse_trm": (_I, [_PM, _I, _I, _I]),
     "mops_d_dinverse_trm": (_I, [_I, _I, _I, _P, _I, _P]),
     "mops_dpotrf": (_I, [_PM, _I, _I, _I]),

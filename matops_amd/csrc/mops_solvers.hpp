@@ -79,6 +79,26 @@ int sytrs_nopiv_d(hipStream_t s, bool lower, int n, int nrhs, const double* A, l
 size_t inverse_trm_ws_bytes(int n);
 int inverse_trm_d(hipStream_t s, bool lower, bool unit, int n, double* A, long lda);
 
+// Symmetric indefinite factorizations with pivoting (sym_piv.hip).
+// ldl_piv_d: A (n x n) = P^T L D L^T P (lower) or P^T U D U^T P in place,
+// 1x1 pivots, the reference's ipiv format (1-based, 0 = no interchange);
+// the npiv entries of ipiv are cleared first. A zero pivot stops it:
+// info (device int, may be null) receives its 0-based index or -1, and
+// err_host (may be null; then the call synchronizes s) 1 + that index or 0.
+// The blocked upper form (nb > 0, n >= nb) synchronizes s once per block.
+// bk_d: Bunch-Kaufman A = L D L^T / U D U^T, LAPACK's ipiv format.
+// ldl_piv_solve_d / bk_solve_d: B (n x nrhs) = A^-1 B from those factors.
+size_t ldl_piv_ws_bytes(int n, int nb, bool upper);
+int ldl_piv_d(hipStream_t s, bool upper, int n, int nb, double* A, long lda, int64_t* ipiv,
+              long npiv, int* info, int* err_host);
+size_t bk_ws_bytes(int n);
+int bk_d(hipStream_t s, bool upper, int n, double* A, long lda, int64_t* ipiv, long npiv);
+size_t ldl_piv_solve_ws_bytes(int n, int nrhs);
+int ldl_piv_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A, long lda,
+                    const int64_t* ipiv, int npiv, double* B, long ldb);
+int bk_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A, long lda,
+               const int64_t* ipiv, double* B, long ldb);
+
 // Cholesky of the N x N matrix A in place (LOWER or UPPER flag), outer block
 // nb. `info` (device int, may be null) receives the 0-based global index of
 // the first NaN square root, or stays -1. `badval` (device double, may be

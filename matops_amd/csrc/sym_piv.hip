@@ -1,0 +1,839 @@
+// Symmetric indefinite factorizations with pivoting, and their solves:
+//
+//   ldl_piv_d   A = P^T L D L^T P or P^T U D U^T P with 1x1 pivots
+//               (DecomposeLDL, ldl.go:25-484): at each step the largest
+//               |diagonal| of the part not yet factored
+//   bk_d        A = L D L^T or U D U^T with Bunch-Kaufman 1x1 / 2x2 pivots
+//               (DecomposeBK, ldlbk.go:107-456, 1114-1140; the algorithm
+//               of LAPACK's dsytf2, with its ipiv format)
+//   ldl_piv_solve_d, bk_solve_d   SolveLDL (ldl.go:486-511) and SolveBK
+//               (ldlbks.go:15-157, 303-316; LAPACK's dsytrs order)
+//
+// Every pivot step is four launches that keep their state in device
+// memory, so the host never waits for a pivot decision:
+//   decide  one workgroup: the pivot search (the reference's IAMax, with
+//           its pair logic and tie rules: calgo/dvec_util.c:164-195) and,
+//           for Bunch-Kaufman, the 1x1 / 2x2 choice; writes the pivot
+//           entries and the next step
+//   swap    the symmetric interchange in the stored triangle, one thread per
+//           swapped pair (pivot.go:145-228 for LDL, which also moves the
+//           rows of the factored columns; ldlbk.go:48-105 and the 2x2 extra
+//           swap for Bunch-Kaufman, which do not)
+//   update  the rank-1 / rank-2 update of the stored triangle of the part
+//           not yet factored, 32 x 32 tiles over the whole GPU
+//   scale   the pivot column(s): l = a / d, or the rows times E^-1
+// A step's size (1 or 2 columns) is known only on the device: the host
+// queues n steps, those past the end return at once, and each update grid
+// is sized for the largest part the step can leave.
+//
+// Blocking. The blocked LOWER LDL (ldl.go:90-295) makes the pivot decisions
+// of the unblocked one on the same updated values when its workspace has
+// nb + 1 columns (its diagonal column then stays apart from the panel
+// columns), so the step algorithm serves every nb; only its zero-pivot
+// check is limited to the columns the reference factors unblocked (the
+// bounded panels divide by zero without an error). Bunch-Kaufman likewise
+// (the blocked form, ldlbk.go:459-1111, makes dsytrf's decisions, which are
+// dsytf2's). The blocked UPPER LDL pivots only inside each diagonal block
+// (ldl.go:356-418), which changes the result: it is kept, with the steps
+// on the block, the interchanges on the blocks beside it, one triangular
+// solve and one triangle-only GEMM per block.
+#include "mops_solvers.hpp"
+
+namespace mops {
+
+namespace {
+
+// The state of the step sequence (device memory).
+struct SymStep {
+  int next;  // LOWER: first column of the next step; UPPER: order of the part left for it
+  int k;     // this step: LOWER its first column; UPPER the order of the part left
+  int r;     // pivot: LOWER relative to k; UPPER absolute, -1 none (Bunch-Kaufman)
+  int np;    // 1 or 2 columns
+  int skip;  // this step does nothing (past the end, or stopped)
+  int err;   // 0, or 1 + the index of a zero pivot (LDL)
+};
+
+constexpr double BK_ALPHA = 0.6403882032022075;  // (1 + sqrt(17)) / 8, ldlbk.go:16
+constexpr int SD_T = 256;                         // threads of the decide kernel
+
+// The reference's IAMax over x[0], x[inc], ... x[(N-1) inc] by one
+// workgroup (every thread gets the result): pair (2q, 2q+1) offers the
+// larger of its two (the first on a tie; a NaN first hides the second) and
+// the first offer strictly above the running maximum (from 0.0) wins; an
+// odd last element likewise. That is the smallest index among the offers
+// of the largest value when it is above zero, else 0.
+__device__ int wg_iamax(const double* __restrict__ x, long inc, int N, double* sv, int* si) {
+  if (N <= 1) return 0;
+  const int t = threadIdx.x;
+  double bv = 0.0;
+  int bi = 0x7fffffff;
+  const int npair = N / 2;
+  for (int q = t; q < npair; q += SD_T) {
+    double c0 = fabs(x[(long)(2 * q) * inc]);
+    const double c1 = fabs(x[(long)(2 * q + 1) * inc]);
+    int i = 2 * q;
+    if (c1 > c0) {
+      c0 = c1;
+      i = 2 * q + 1;
+    }
+    if (c0 > bv) {
+      bv = c0;
+      bi = i;
+    }
+  }
+  if ((N & 1) && t == SD_T - 1) {
+    // the last offer; it wins only above every pair's (ties go to the lower
+    // index in the reduction below)
+    const double c0 = fabs(x[(long)(N - 1) * inc]);
+    if (c0 > bv) {
+      bv = c0;
+      bi = N - 1;
+    }
+  }
+  sv[t] = bv;
+  si[t] = bi;
+  __syncthreads();
+  for (int h = SD_T / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      const double v = sv[t + h];
+      const int i = si[t + h];
+      if (v > sv[t] || (v == sv[t] && i < si[t])) {
+        sv[t] = v;
+        si[t] = i;
+      }
+    }
+    __syncthreads();
+  }
+  const int r = si[0];
+  __syncthreads();  // sv / si free for the next search
+  return r == 0x7fffffff ? 0 : r;
+}
+
+// A[i][j] of a column-major matrix
+__device__ __forceinline__ double& at(double* A, long lda, int i, int j) {
+  return A[i + (long)j * lda];
+}
+__device__ __forceinline__ double at(const double* A, long lda, int i, int j) {
+  return A[i + (long)j * lda];
+}
+
+__device__ __forceinline__ void swap_el(double* a, double* b) {
+  const double v = *a;
+  *a = *b;
+  *b = v;
+}
+
+__global__ void k_sym_init(SymStep* st, int next) {
+  st->next = next;
+  st->skip = 0;
+  st->err = 0;
+}
+
+__global__ void k_fill_i64(int64_t* p, long n, int64_t v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// info = the 0-based index of the zero pivot, -1 when none
+__global__ void k_sym_info(const SymStep* st, int* info) { *info = st->err - 1; }
+
+// The pivot decision of one step on the n x n view A; ipiv: the view's
+// pivot entries. base: added to a zero pivot's index; LOWER LDL checks for
+// a zero pivot only in the columns from check_from on.
+template <bool BK, bool UPPER>
+__global__ __launch_bounds__(SD_T) void k_sym_decide(int n, const double* __restrict__ A, long lda,
+                                                     SymStep* st, int64_t* ipiv, int base,
+                                                     int check_from) {
+  __shared__ double sv[SD_T];
+  __shared__ int si[SD_T];
+  const int t = threadIdx.x;
+  const int k = st->next;
+  const bool end = st->err != 0 || (UPPER ? k <= 0 : k >= n);
+  __syncthreads();  // every thread has read the state before thread 0 writes it
+  if (end) {
+    if (t == 0) st->skip = 1;
+    return;
+  }
+  int r = 0, np = 1, err = 0;
+  if (!BK) {
+    if (!UPPER) {
+      // ldl.go:47-64: the largest |diagonal| of the trailing part
+      r = wg_iamax(A + k + (long)k * lda, lda + 1, n - k, sv, si);
+      if (t == 0) ipiv[k] = r > 0 ? r + k + 1 : 0;
+      if (k >= check_from && at(A, lda, k + r, k + r) == 0.0) err = 1 + base + k;
+    } else {
+      // ldl.go:321-333: the largest |diagonal| of the leading part
+      r = wg_iamax(A, lda + 1, k, sv, si);
+      if (t == 0) ipiv[k - 1] = r < k - 1 ? r + 1 : 0;
+      if (at(A, lda, r, r) == 0.0) err = base + k;  // 1 + index k - 1
+    }
+  } else if (!UPPER) {
+    // findBKPivot, LOWER (ldlbk.go:109-159) on T = A[k:, k:]
+    const int m = n - k;
+    const double* T = A + k + (long)k * lda;
+    if (m > 1) {
+      const double amax = fabs(T[0]);
+      const int rr = wg_iamax(T + 1, 1, m - 1, sv, si) + 1;
+      const double rmax = fabs(T[rr]);
+      if (!(amax >= BK_ALPHA * rmax)) {
+        // the largest off-diagonal |element| of row / column rr
+        const int q = wg_iamax(T + rr, lda, rr, sv, si);
+        double qmax = fabs(at(T, lda, rr, q));
+        if (rr < m - 1) {
+          const double* col = T + (rr + 1) + (long)rr * lda;
+          const double qmax2 = fabs(col[wg_iamax(col, 1, m - rr - 1, sv, si)]);
+          if (qmax2 > qmax) qmax = qmax2;
+        }
+        if (!(amax >= BK_ALPHA * rmax * (rmax / qmax))) {
+          r = rr;
+          np = fabs(at(T, lda, rr, rr)) >= BK_ALPHA * qmax ? 1 : 2;
+        }
+      }
+    }
+    if (t == 0) {
+      ipiv[k] = np == 1 ? r + k + 1 : -(r + k + 1);
+      if (np == 2) ipiv[k + 1] = -(r + k + 1);
+    }
+  } else {
+    // findBKPivot, UPPER (ldlbk.go:161-208) on A[0:e, 0:e]
+    const int e = k, last = e - 1;
+    r = e == 1 ? 0 : -1;
+    if (e > 1) {
+      const double amax = fabs(at(A, lda, last, last));
+      const int rr = wg_iamax(A + (long)last * lda, 1, last, sv, si);
+      const double rmax = fabs(at(A, lda, rr, last));
+      if (!(amax >= BK_ALPHA * rmax)) {
+        double qmax = 0.0;
+        if (rr > 0) qmax = fabs(at(A, lda, wg_iamax(A + (long)rr * lda, 1, rr, sv, si), rr));
+        const double* row = A + rr + (long)(rr + 1) * lda;
+        const double qmax2 = fabs(row[(long)wg_iamax(row, lda, last - rr, sv, si) * lda]);
+        if (qmax2 > qmax) qmax = qmax2;
+        if (!(amax >= BK_ALPHA * rmax * (rmax / qmax))) {
+          r = rr;
+          np = fabs(at(A, lda, rr, rr)) >= BK_ALPHA * qmax ? 1 : 2;
+        }
+      }
+    }
+    if (t == 0) {
+      if (np == 1) {
+        ipiv[e - 1] = r == -1 ? e : r + 1;
+      } else {
+        ipiv[e - 2] = -(r + 1);
+        ipiv[e - 1] = -(r + 1);
+      }
+    }
+  }
+  if (t == 0) {
+    st->k = k;
+    st->r = r;
+    st->np = np;
+    st->err = err;
+    st->skip = 0;  // the interchange of a zero-pivot step is still made
+    st->next = UPPER ? k - np : k + np;
+  }
+}
+
+// The symmetric interchange of the step, one thread per swapped pair (the
+// pairs of one interchange are disjoint).
+template <bool BK, bool UPPER>
+__global__ void k_sym_swap(int n, double* A, long lda, const SymStep* st) {
+  if (st->skip) return;
+  const int k = st->k, r = st->r, np = st->np;
+  const long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  if (!BK && !UPPER) {
+    // applyPivotSym LOWER (pivot.go:157-178), AL = A[k:, :k], AR = A[k:, k:]
+    if (r == 0) return;
+    const int p = k + r;  // the pivot row
+    const long n1 = k, n2 = r - 1, n3 = n - p - 1;
+    for (long e = e0; e < n1 + n2 + n3 + 1; e += stride) {
+      if (e < n1) {  // rows k and p of the factored columns
+        swap_el(&at(A, lda, k, (int)e), &at(A, lda, p, (int)e));
+      } else if (e < n1 + n2) {  // column k between the two <-> row p
+        const int i = k + 1 + (int)(e - n1);
+        swap_el(&at(A, lda, i, k), &at(A, lda, p, i));
+      } else if (e < n1 + n2 + n3) {  // columns k and p below row p
+        const int i = p + 1 + (int)(e - n1 - n2);
+        swap_el(&at(A, lda, i, k), &at(A, lda, i, p));
+      } else {
+        swap_el(&at(A, lda, k, k), &at(A, lda, p, p));
+      }
+    }
+    return;
+  }
+  if (!BK && UPPER) {
+    // applyPivotSym UPPER (pivot.go:179-200), AL = A[0:e, :], AR = A[e-1, e-1:]
+    const int e = k, c = e - 1, idx = r;
+    if (idx >= c) return;
+    const long n1 = n - e, n2 = e - idx - 2, n3 = idx;
+    for (long q = e0; q < n1 + n2 + n3 + 1; q += stride) {
+      if (q < n1) {  // rows c and idx right of the part left
+        const int j = e + (int)q;
+        swap_el(&at(A, lda, c, j), &at(A, lda, idx, j));
+      } else if (q < n1 + n2) {  // column c between the two <-> row idx
+        const int i = idx + 1 + (int)(q - n1);
+        swap_el(&at(A, lda, i, c), &at(A, lda, idx, i));
+      } else if (q < n1 + n2 + n3) {  // columns idx and c above row idx
+        const int i = (int)(q - n1 - n2);
+        swap_el(&at(A, lda, i, idx), &at(A, lda, i, c));
+      } else {
+        swap_el(&at(A, lda, c, c), &at(A, lda, idx, idx));
+      }
+    }
+    return;
+  }
+  if (BK && !UPPER) {
+    // ldlbk.go:228-243: applyBKPivotSym(T = A[k:, k:], np - 1, r) unless r
+    // is 0 or np - 1, and for a 2x2 pivot T[1][0] <-> T[r][0]
+    if (r == 0 || r == np - 1) return;
+    double* T = A + k + (long)k * lda;
+    const int m = n - k, src = np - 1, dst = r;
+    const long n2 = dst - src - 1, n3 = m - dst - 1, nx = np == 2 ? 1 : 0;
+    for (long e = e0; e < n2 + n3 + 1 + nx; e += stride) {
+      if (e < n2) {
+        const int i = src + 1 + (int)e;
+        swap_el(&at(T, lda, i, src), &at(T, lda, dst, i));
+      } else if (e < n2 + n3) {
+        const int i = dst + 1 + (int)(e - n2);
+        swap_el(&at(T, lda, i, src), &at(T, lda, i, dst));
+      } else if (e == n2 + n3) {
+        swap_el(&at(T, lda, src, src), &at(T, lda, dst, dst));
+      } else {
+        swap_el(&at(T, lda, 1, 0), &at(T, lda, r, 0));
+      }
+    }
+    return;
+  }
+  // BK UPPER (ldlbk.go:359-374): applyBKPivotSym(A[0:e, 0:e], e - np, r)
+  // unless r is -1, and for a 2x2 pivot A[e-2][e-1] <-> A[r][e-1]
+  if (r == -1) return;
+  const int e = k, src = e - np, dst = r, nr = e - 1;
+  if (src == dst && np == 1) return;
+  const long n2 = src > dst ? src - dst - 1 : 0, n3 = src > dst ? dst : 0;
+  const long nd = src > dst ? 1 : 0, nx = np == 2 && r != nr - 1 ? 1 : 0;
+  for (long q = e0; q < n2 + n3 + nd + nx; q += stride) {
+    if (q < n2) {
+      const int i = dst + 1 + (int)q;
+      swap_el(&at(A, lda, i, src), &at(A, lda, dst, i));
+    } else if (q < n2 + n3) {
+      const int i = (int)(q - n2);
+      swap_el(&at(A, lda, i, src), &at(A, lda, i, dst));
+    } else if (q < n2 + n3 + nd) {
+      swap_el(&at(A, lda, src, src), &at(A, lda, dst, dst));
+    } else {
+      swap_el(&at(A, lda, nr - 1, nr), &at(A, lda, r, nr));
+    }
+  }
+}
+
+// The update of the part not yet factored, on its stored triangle: LOWER
+// A22 = A[k+np:, k+np:], UPPER A00 = A[0:e-np, 0:e-np]; block (x, y) is the
+// 32 x 32 tile of rows 32 y, columns 32 x of that part.
+//   np = 1: A22[i][j] += x_i (alpha x_j), x the pivot column, alpha = -1/d
+//           (MVUpdateTrm)
+//   np = 2: A22[i][j] -= l_i0 c_j0 + l_i1 c_j1, c the two pivot columns and
+//           l = c E^-1 their scaled form (ldlbk.go:287-298)
+template <bool UPPER>
+__global__ __launch_bounds__(256) void k_sym_update(int n, double* A, long lda,
+                                                    const SymStep* st) {
+  if (st->skip || st->err) return;
+  const int k = st->k, np = st->np;
+  const int m = UPPER ? k - np : n - k - np;  // order of the updated part
+  const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;
+  if (bi >= m || bj >= m) return;
+  if (!UPPER && bj > bi + 31) return;  // above the diagonal
+  if (UPPER && bi > bj + 31) return;   // below the diagonal
+  const int o = UPPER ? 0 : k + np;    // first row / column of the part
+  const int c0 = UPPER ? k - np : k;   // first pivot column
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i = bi + tx;
+  if (i >= m) return;
+  if (np == 1) {
+    const double alpha = -1.0 / at(A, lda, c0, c0);
+    const double xi = at(A, lda, o + i, c0);
+    for (int q = ty; q < 32; q += 8) {
+      const int j = bj + q;
+      if (j >= m || (!UPPER && i < j) || (UPPER && i > j)) continue;
+      at(A, lda, o + i, o + j) += xi * (alpha * at(A, lda, o + j, c0));
+    }
+    return;
+  }
+  // the 2 x 2 block E = [a b; b d] at rows / columns c0, c0 + 1; its
+  // inverse is scale [d/b -1; -1 a/b], scale = 1 / ((a/b)(d/b) - 1) / b
+  const double a = at(A, lda, c0, c0), d = at(A, lda, c0 + 1, c0 + 1);
+  const double b = UPPER ? at(A, lda, c0, c0 + 1) : at(A, lda, c0 + 1, c0);
+  const double i00 = d / b, i11 = a / b;
+  const double scale = 1.0 / ((a / b) * (d / b) - 1.0) / b;
+  const double ci0 = at(A, lda, o + i, c0), ci1 = at(A, lda, o + i, c0 + 1);
+  const double l0 = scale * (ci0 * i00 + ci1 * -1.0);
+  const double l1 = scale * (ci0 * -1.0 + ci1 * i11);
+  for (int q = ty; q < 32; q += 8) {
+    const int j = bj + q;
+    if (j >= m || (!UPPER && i < j) || (UPPER && i > j)) continue;
+    at(A, lda, o + i, o + j) -= l0 * at(A, lda, o + j, c0) + l1 * at(A, lda, o + j, c0 + 1);
+  }
+}
+
+// The pivot column(s) of the step: a / d (InvScale divides), or each row
+// of the two columns times E^-1.
+template <bool UPPER>
+__global__ void k_sym_scale(int n, double* A, long lda, const SymStep* st) {
+  if (st->skip || st->err) return;
+  const int k = st->k, np = st->np;
+  const int m = UPPER ? k - np : n - k - np;
+  const int o = UPPER ? 0 : k + np;
+  const int c0 = UPPER ? k - np : k;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    if (np == 1) {
+      at(A, lda, o + i, c0) /= at(A, lda, c0, c0);
+    } else {
+      const double a = at(A, lda, c0, c0), d = at(A, lda, c0 + 1, c0 + 1);
+      const double b = UPPER ? at(A, lda, c0, c0 + 1) : at(A, lda, c0 + 1, c0);
+      const double i00 = d / b, i11 = a / b;
+      const double scale = 1.0 / ((a / b) * (d / b) - 1.0) / b;
+      const double ci0 = at(A, lda, o + i, c0), ci1 = at(A, lda, o + i, c0 + 1);
+      at(A, lda, o + i, c0) = scale * (ci0 * i00 + ci1 * -1.0);
+      at(A, lda, o + i, c0 + 1) = scale * (ci0 * -1.0 + ci1 * i11);
+    }
+  }
+}
+
+inline int grid_for(long total) {
+  const long b = (total + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+// The n steps of the unblocked algorithm on the n x n view A (state st
+// already initialised).
+template <bool BK, bool UPPER>
+int sym_steps(hipStream_t s, int n, double* A, long lda, int64_t* ipiv, SymStep* st, int base,
+              int check_from) {
+  for (int i = 0; i < n; ++i) {
+    hipLaunchKernelGGL((k_sym_decide<BK, UPPER>), dim3(1), dim3(SD_T), 0, s, n, A, lda, st, ipiv,
+                       base, check_from);
+    hipLaunchKernelGGL((k_sym_swap<BK, UPPER>), dim3(grid_for(n + 2)), dim3(256), 0, s, n, A, lda,
+                       st);
+    const int mmax = n - i - 1;  // the order of the part this step can leave
+    if (mmax > 0) {
+      const int tiles = ceil_div(mmax, 32);
+      hipLaunchKernelGGL((k_sym_update<UPPER>), dim3(tiles, tiles), dim3(256), 0, s, n, A, lda,
+                         st);
+      hipLaunchKernelGGL((k_sym_scale<UPPER>), dim3(grid_for(mmax)), dim3(256), 0, s, n, A, lda,
+                         st);
+    }
+    MOPS_CHECK_HIP(hipGetLastError());
+  }
+  return ST_OK;
+}
+
+// After a zero pivot in the columns from c0 on of the blocked LOWER LDL:
+// the reference returns from its unblocked tail (ldl.go:280-285) before the
+// tail's interchanges reach the rows of columns 0..c0-1 and before its
+// pivots are made global. Undoes the first and the second of the step
+// algorithm for the tail's steps up to the zero pivot. One workgroup.
+__global__ __launch_bounds__(256) void k_ldl_tail_fixup(const SymStep* st, double* A, long lda,
+                                                        int64_t* ipiv, int c0) {
+  const int e = st->err - 1;  // the zero pivot's column
+  if (e < c0) return;
+  for (int c = threadIdx.x; c < c0; c += 256)
+    for (int q = e; q >= c0; --q)
+      if (ipiv[q] > 0) swap_el(&at(A, lda, q, c), &at(A, lda, (int)ipiv[q] - 1, c));
+  __syncthreads();
+  for (int q = c0 + threadIdx.x; q <= e; q += 256)
+    if (ipiv[q] > 0) ipiv[q] -= c0;
+}
+
+// The interchanges of a block's pivots p[0..kb) (1-based within the block,
+// 0 = none), last to first (BACKWARD, pivot.go:63-97), on the columns
+// (on_cols) or the rows of the rows x cols matrix X; one thread per row
+// (on_cols) or column of X.
+__global__ void k_block_pivots(int rows, int cols, double* X, long ldx, const int64_t* p, int kb,
+                               int on_cols) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (on_cols) {
+    if (t >= rows) return;
+    for (int q = kb - 1; q >= 0; --q)
+      if (p[q] > 0) swap_el(&X[t + (long)q * ldx], &X[t + (long)(p[q] - 1) * ldx]);
+  } else {
+    if (t >= cols) return;
+    double* x = X + (long)t * ldx;
+    for (int q = kb - 1; q >= 0; --q)
+      if (p[q] > 0) swap_el(&x[q], &x[p[q] - 1]);
+  }
+}
+
+__global__ void k_shift_pivots(int64_t* p, int kb, int off) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < kb && p[t] > 0) p[t] += off;
+}
+
+// X (rows x kb) columns times 1 / d_j (SolveDiag RIGHT), and W = X D
+// (MultDiag RIGHT), d_j = A[j][j]
+__global__ void k_diag_right(int rows, int kb, double* X, long ldx, const double* A, long lda,
+                             double* W, long ldw) {
+  const long total = (long)rows * kb;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    const double d = A[(long)j * (lda + 1)];
+    const double x = X[i + (long)j * ldx] * (1.0 / d);
+    X[i + (long)j * ldx] = x;
+    W[i + (long)j * ldw] = x * d;
+  }
+}
+
+// perm = the row order after the interchanges rows k <-> p[k] - 1 for the
+// entries p[k] > 0 of p[0..npv), in order (forward) or last to first:
+// B'[i] = B[perm[i]]. Entries beyond n are ignored. One workgroup; the
+// sequence runs in LDS when it fits.
+constexpr int PERM_LDS = 32768;
+__global__ __launch_bounds__(256) void k_pivots_to_perm(int n, int npv,
+                                                        const int64_t* __restrict__ p,
+                                                        int forward, int* perm) {
+  __shared__ int q[PERM_LDS];
+  const bool lds = n <= PERM_LDS;
+  int* v = lds ? q : perm;
+  for (int i = threadIdx.x; i < n; i += 256) v[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < npv; ++s) {
+      const int k = forward ? s : npv - 1 - s;
+      const long pk = p[k];
+      if (pk > 0 && pk <= n) {
+        const int o = (int)pk - 1, t = v[k];
+        v[k] = v[o];
+        v[o] = t;
+      }
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (int i = threadIdx.x; i < n; i += 256) perm[i] = q[i];
+}
+
+// gather: Y[i][j] = X[perm[i]][j]; else scatter: Y[perm[i]][j] = X[i][j]
+__global__ void k_permute_rows(int n, int nrhs, const int* __restrict__ perm, const double* X,
+                               long ldx, double* Y, long ldy, int gather) {
+  const long total = (long)n * nrhs;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (gather)
+      Y[i + (long)j * ldy] = X[perm[i] + (long)j * ldx];
+    else
+      Y[perm[i] + (long)j * ldy] = X[i + (long)j * ldx];
+  }
+}
+
+// SolveDiag(B, A, LEFT) (diag.go:85-139): rows times 1 / d_k; a 1 x 1 A
+// divides
+__global__ void k_solve_diag_left(int n, int nrhs, double* B, long ldb, const double* A,
+                                  long lda) {
+  const long total = (long)n * nrhs;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+       e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (n == 1)
+      B[(long)j * ldb] /= A[0];
+    else
+      B[i + (long)j * ldb] *= 1.0 / A[(long)i * (lda + 1)];
+  }
+}
+
+// Sum over the 256 threads; every thread gets it
+__device__ double wg_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// The 2 x 2 block solve of SolveBK (ldlbks.go:62-80): [b0 b1] = E^-1 [b0 b1]
+__device__ __forceinline__ void solve_e(double a, double b, double d, double* b0, double* b1) {
+  const double apb = a / b, dpb = d / b;
+  const double scale = (apb * dpb - 1.0) * b;
+  const double s0 = *b0, s1 = *b1;
+  *b0 = (dpb * s0 - s1) / scale;
+  *b1 = (apb * s1 - s0) / scale;
+}
+
+// SolveBK on one right-hand side column per workgroup, in LAPACK dsytrs
+// order. LOWER (ldlbks.go:15-157): blocks first to last, the interchange,
+// the rows below -= the block's columns times b1, b1 = D^-1 b1; then last
+// to first, b1 -= the block's columns^T times the rows below, the
+// interchange. UPPER: the mirror image (blocks last to first, then first to
+// last, the rows above).
+// b[i] <-> b[j]; a pivot entry out of range is ignored
+__device__ __forceinline__ void swap_rows(double* b, int i, long j, int n) {
+  if (j >= 0 && j < n) swap_el(&b[i], &b[j]);
+}
+
+template <bool UPPER>
+__global__ __launch_bounds__(256) void k_bk_solve(int n, const double* __restrict__ A, long lda,
+                                                  const int64_t* __restrict__ p, double* B,
+                                                  long ldb) {
+  __shared__ double red[4];
+  double* b = B + (long)blockIdx.x * ldb;
+  const int t = threadIdx.x;
+  if (!UPPER) {
+    for (int k = 0; k < n;) {
+      const int np = p[k] < 0 && k + 1 < n ? 2 : 1;
+      if (t == 0) {
+        if (np == 1) {
+          if (p[k] != k + 1) swap_rows(b, k, p[k] - 1, n);
+        } else if (-p[k] != k + 2) {
+          swap_rows(b, k + 1, -p[k] - 1, n);
+        }
+      }
+      __syncthreads();
+      const double b0 = b[k], b1 = np == 2 ? b[k + 1] : 0.0;
+      for (int i = k + np + t; i < n; i += 256) {
+        double v = b[i] - at(A, lda, i, k) * b0;
+        if (np == 2) v -= at(A, lda, i, k + 1) * b1;
+        b[i] = v;
+      }
+      if (t == 0) {
+        if (np == 1) {
+          b[k] = b0 / at(A, lda, k, k);
+        } else {
+          double x0 = b0, x1 = b1;
+          solve_e(at(A, lda, k, k), at(A, lda, k + 1, k), at(A, lda, k + 1, k + 1), &x0, &x1);
+          b[k] = x0;
+          b[k + 1] = x1;
+        }
+      }
+      __syncthreads();
+      k += np;
+    }
+    for (int nc = n; nc > 0;) {
+      const int np = p[nc - 1] < 0 && nc > 1 ? 2 : 1, k = nc - np;
+      double s0 = 0.0, s1 = 0.0;
+      for (int i = nc + t; i < n; i += 256) {
+        s0 += at(A, lda, i, k) * b[i];
+        if (np == 2) s1 += at(A, lda, i, k + 1) * b[i];
+      }
+      s0 = wg_sum(s0, red);
+      if (np == 2) s1 = wg_sum(s1, red);
+      if (t == 0) {
+        b[k] -= s0;
+        if (np == 2) b[k + 1] -= s1;
+        if (np == 1) {
+          if (p[k] != nc) swap_rows(b, k, p[k] - 1, n);
+        } else if (-p[k] != nc) {
+          swap_rows(b, k + 1, -p[k] - 1, n);
+        }
+      }
+      __syncthreads();
+      nc -= np;
+    }
+    return;
+  }
+  for (int kk = n; kk > 0;) {  // kk: 1-based index of the block's last row
+    const int np = p[kk - 1] < 0 && kk > 1 ? 2 : 1, k = kk - np;
+    if (t == 0) {
+      if (np == 1) {
+        if (p[k] != kk) swap_rows(b, k, p[k] - 1, n);
+      } else if (-p[k] != kk - 1) {
+        swap_rows(b, k, -p[k] - 1, n);
+      }
+    }
+    __syncthreads();
+    const double b0 = b[k], b1 = np == 2 ? b[k + 1] : 0.0;
+    for (int i = t; i < k; i += 256) {
+      double v = b[i] - at(A, lda, i, k) * b0;
+      if (np == 2) v -= at(A, lda, i, k + 1) * b1;
+      b[i] = v;
+    }
+    if (t == 0) {
+      if (np == 1) {
+        b[k] = b0 / at(A, lda, k, k);
+      } else {
+        double x0 = b0, x1 = b1;
+        solve_e(at(A, lda, k, k), at(A, lda, k, k + 1), at(A, lda, k + 1, k + 1), &x0, &x1);
+        b[k] = x0;
+        b[k + 1] = x1;
+      }
+    }
+    __syncthreads();
+    kk -= np;
+  }
+  for (int k = 0; k < n;) {
+    const int np = p[k] < 0 && k + 1 < n ? 2 : 1;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = t; i < k; i += 256) {
+      s0 += at(A, lda, i, k) * b[i];
+      if (np == 2) s1 += at(A, lda, i, k + 1) * b[i];
+    }
+    s0 = wg_sum(s0, red);
+    if (np == 2) s1 = wg_sum(s1, red);
+    if (t == 0) {
+      b[k] -= s0;
+      if (np == 2) b[k + 1] -= s1;
+      if (np == 1) {
+        if (p[k] != k + 1) swap_rows(b, k, p[k] - 1, n);
+      } else if (-p[k] != k + 1) {
+        swap_rows(b, k, -p[k] - 1, n);
+      }
+    }
+    __syncthreads();
+    k += np;
+  }
+}
+
+}  // namespace
+
+size_t ldl_piv_ws_bytes(int n, int nb, bool upper) {
+  if (n <= 0) return 0;
+  size_t b = 1024;
+  if (upper && nb > 0 && n >= nb)
+    b += (size_t)n * nb * sizeof(double) + trsm_ws_bytes(F_RIGHT, nb, n) +
+         gemm_split_ws_bytes(n, n, nb, sizeof(double)) + 1024;
+  return b;
+}
+
+int ldl_piv_d(hipStream_t s, bool upper, int n, int nb, double* A, long lda, int64_t* ipiv,
+              long npiv, int* info, int* err_host) {
+  if (err_host) *err_host = 0;
+  if (npiv > 0) {
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(npiv)), dim3(256), 0, s, ipiv, npiv, (int64_t)0);
+    MOPS_CHECK_HIP(hipGetLastError());
+  }
+  SymStep* st = (SymStep*)ws_carve(sizeof(SymStep));
+  if (!st) {
+    set_error("ldl: workspace too small");
+    return ST_NOMEM;
+  }
+  const bool blocked = !(n < nb || nb == 0);
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, 0);
+  } else if (!upper) {
+    // the columns the blocked form factors unblocked: from the first
+    // multiple of nb that leaves at most nb columns
+    int check_from = 0;
+    if (blocked) {
+      int k = 0;
+      while (n - k > nb) k += nb;
+      check_from = k;
+    }
+    hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, 0);
+    MOPS_TRY((sym_steps<false, false>(s, n, A, lda, ipiv, st, 0, check_from)));
+    if (check_from > 0) {
+      hipLaunchKernelGGL(k_ldl_tail_fixup, dim3(1), dim3(256), 0, s, st, A, lda, ipiv, check_from);
+      MOPS_CHECK_HIP(hipGetLastError());
+    }
+  } else if (!blocked) {
+    hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, n);
+    MOPS_TRY((sym_steps<false, true>(s, n, A, lda, ipiv, st, 0, 0)));
+  } else {
+    // blkUpperLDL (ldl.go:356-418): pivots inside each diagonal block; the
+    // host reads the state after each block to stop where the reference
+    // returns on a zero pivot
+    double* W = (double*)ws_carve((size_t)n * nb * sizeof(double));
+    if (!W) {
+      set_error("ldl: workspace too small");
+      return ST_NOMEM;
+    }
+    for (int e = n; e > 0;) {
+      const int kb = e < nb ? e : nb, k = e - kb;
+      double* A11 = A + k + (long)k * lda;
+      hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, kb);
+      MOPS_TRY((sym_steps<false, true>(s, kb, A11, lda, ipiv + k, st, k, 0)));
+      SymStep h;
+      MOPS_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(SymStep), hipMemcpyDeviceToHost, s));
+      MOPS_CHECK_HIP(hipStreamSynchronize(s));
+      if (h.err) break;
+      if (k > 0)
+        hipLaunchKernelGGL(k_block_pivots, dim3(ceil_div(k, 256)), dim3(256), 0, s, k, kb,
+                           A + (long)k * lda, lda, ipiv + k, kb, 1);
+      if (n - e > 0)
+        hipLaunchKernelGGL(k_block_pivots, dim3(ceil_div(n - e, 256)), dim3(256), 0, s, kb, n - e,
+                           A + k + (long)e * lda, lda, ipiv + k, kb, 0);
+      hipLaunchKernelGGL(k_shift_pivots, dim3(ceil_div(kb, 256)), dim3(256), 0, s, ipiv + k, kb, k);
+      MOPS_CHECK_HIP(hipGetLastError());
+      if (k > 0) {
+        double* A01 = A + (long)k * lda;
+        MOPS_TRY(trsm_d(s, F_RIGHT | F_UPPER | F_UNIT | F_TRANSA, kb, k, 1.0, A11, lda, A01, lda));
+        hipLaunchKernelGGL(k_diag_right, dim3(grid_for((long)k * kb)), dim3(256), 0, s, k, kb, A01,
+                           lda, A11, lda, W, (long)k);
+        MOPS_CHECK_HIP(hipGetLastError());
+        MOPS_TRY(gemm_d(s, false, true, k, k, kb, -1.0, A01, lda, W, k, 1.0, A, lda, GM_UPPER));
+      }
+      e = k;
+    }
+  }
+  if (info) hipLaunchKernelGGL(k_sym_info, dim3(1), dim3(1), 0, s, st, info);
+  if (err_host) {
+    SymStep h;
+    MOPS_CHECK_HIP(hipMemcpyAsync(&h, st, sizeof(SymStep), hipMemcpyDeviceToHost, s));
+    MOPS_CHECK_HIP(hipStreamSynchronize(s));
+    *err_host = h.err;
+  }
+  return ST_OK;
+}
+
+size_t bk_ws_bytes(int n) { return n > 0 ? 1024 : 0; }
+
+int bk_d(hipStream_t s, bool upper, int n, double* A, long lda, int64_t* ipiv, long npiv) {
+  if (npiv > 0) {
+    hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(npiv)), dim3(256), 0, s, ipiv, npiv, (int64_t)0);
+    MOPS_CHECK_HIP(hipGetLastError());
+  }
+  if (n <= 0) return ST_OK;
+  SymStep* st = (SymStep*)ws_carve(sizeof(SymStep));
+  if (!st) {
+    set_error("bk: workspace too small");
+    return ST_NOMEM;
+  }
+  hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, upper ? n : 0);
+  if (upper) return sym_steps<true, true>(s, n, A, lda, ipiv, st, 0, 0);
+  return sym_steps<true, false>(s, n, A, lda, ipiv, st, 0, 0);
+}
+
+size_t ldl_piv_solve_ws_bytes(int n, int nrhs) {
+  if (n <= 0 || nrhs <= 0) return 0;
+  return (size_t)n * nrhs * sizeof(double) + (size_t)n * sizeof(int) +
+         trsm_ws_bytes(F_LEFT, n, nrhs) + 1024;
+}
+
+int ldl_piv_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A, long lda,
+                    const int64_t* ipiv, int npiv, double* B, long ldb) {
+  if (n <= 0 || nrhs <= 0) return ST_OK;
+  // SolveLDL (ldl.go:486-511): B = P B (the interchanges in order for LOWER,
+  // last to first for UPPER), the unit-triangle solve, the diagonal, the
+  // transposed unit-triangle solve, then B = P^T B. P is applied as one
+  // gather into a workspace copy and undone as one scatter back into B.
+  double* W = (double*)ws_carve((size_t)n * nrhs * sizeof(double));
+  int* perm = (int*)ws_carve((size_t)n * sizeof(int));
+  if (!W || !perm) {
+    set_error("ldl solve: workspace too small");
+    return ST_NOMEM;
+  }
+  const int np = npiv < n ? npiv : n;
+  hipLaunchKernelGGL(k_pivots_to_perm, dim3(1), dim3(256), 0, s, n, np, ipiv, upper ? 0 : 1, perm);
+  const int g = grid_for((long)n * nrhs);
+  hipLaunchKernelGGL(k_permute_rows, dim3(g), dim3(256), 0, s, n, nrhs, perm, B, ldb, W, (long)n, 1);
+  MOPS_CHECK_HIP(hipGetLastError());
+  const int tri = upper ? F_UPPER : F_LOWER;
+  MOPS_TRY(trsm_d(s, F_LEFT | tri | F_UNIT, n, nrhs, 1.0, A, lda, W, n));
+  hipLaunchKernelGGL(k_solve_diag_left, dim3(g), dim3(256), 0, s, n, nrhs, W, (long)n, A, lda);
+  MOPS_CHECK_HIP(hipGetLastError());
+  MOPS_TRY(trsm_d(s, F_LEFT | tri | F_UNIT | F_TRANSA, n, nrhs, 1.0, A, lda, W, n));
+  hipLaunchKernelGGL(k_permute_rows, dim3(g), dim3(256), 0, s, n, nrhs, perm, W, (long)n, B, ldb, 0);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+int bk_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A, long lda,
+               const int64_t* ipiv, double* B, long ldb) {
+  if (n <= 0 || nrhs <= 0) return ST_OK;
+  if (upper)
+    hipLaunchKernelGGL(k_bk_solve<true>, dim3(nrhs), dim3(256), 0, s, n, A, lda, ipiv, B, ldb);
+  else
+    hipLaunchKernelGGL(k_bk_solve<false>, dim3(nrhs), dim3(256), 0, s, n, A, lda, ipiv, B, ldb);
+  MOPS_CHECK_HIP(hipGetLastError());
+  return ST_OK;
+}
+
+}  // namespace mops

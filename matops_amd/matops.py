@@ -434,6 +434,90 @@ def SolveLDLnoPiv(B, A, flags):
         ctypes.byref(Bm), ctypes.byref(Am), flags, n, B.shape[1]))
 
 
+def _pivot_array(ipiv):
+    if not isinstance(ipiv, np.ndarray) or ipiv.dtype != np.int64 \
+            or not ipiv.flags.c_contiguous or ipiv.ndim != 1:
+        raise TypeError("ipiv must be a contiguous 1-D int64 array (Go []int)")
+    return ctypes.c_void_p(ipiv.ctypes.data) if len(ipiv) else None
+
+
+def DecomposeLDL(A, W, ipiv, flags, nb):
+    """A = P^T L D L^T P (LOWER) or P^T U D U^T P with 1x1 pivots, in place
+    (ldl.go:439-460). W: the reference's workspace for the blocked forms
+    (not read; the GPU keeps its own). ipiv: int64 array of N entries,
+    1-based, 0 = no interchange. Returns (A, err); a zero pivot gives the
+    reference's error, or raises with SetPanicOnError (ldl.go:60, 335)."""
+    if A.shape[0] != A.shape[1]:
+        return None, MatopsError("A not a square matrix")
+    p = _pivot_array(ipiv)
+    ipiv[:] = 0
+    n = A.shape[0]
+    if n == 0:
+        return A, None
+    if len(ipiv) < n:
+        raise IndexError("index out of range")  # the reference's slice access
+    Am, _, _ = _view(A)
+    rc = _lib.lib().mops_dsytrf_ldl(ctypes.byref(Am), p, len(ipiv), flags, n, nb)
+    if rc > 0:
+        return A, _onError(_lib.last_error())
+    check("mops_dsytrf_ldl", rc)
+    return A, None
+
+
+def SolveLDL(B, A, ipiv, flags):
+    """B = A^-1 B from DecomposeLDL's factors and pivots (ldl.go:486-511);
+    UPPER is tested first, neither bit leaves B as is."""
+    n = A.shape[0]
+    if n == 0 or B.shape[1] == 0:
+        return
+    piv = np.ascontiguousarray(ipiv, dtype=np.int64)
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dsytrs_ldl", _lib.lib().mops_dsytrs_ldl(
+        ctypes.byref(Bm), ctypes.byref(Am), _pivot_array(piv), len(piv), flags, n,
+        B.shape[1]))
+
+
+def DecomposeBK(A, W, ipiv, flags, nb):
+    """A = L D L^T (LOWER) or U D U^T (UPPER) with Bunch-Kaufman 1x1 / 2x2
+    pivots, in place (ldlbk.go:1114-1140). W: the reference's workspace,
+    checked as it does (N x 2 unblocked, N x (nb + 1) blocked; not read).
+    ipiv: int64 array, LAPACK's format (1-based; negative for 2x2 blocks).
+    Returns (A, err)."""
+    p = _pivot_array(ipiv)
+    ipiv[:] = 0
+    n = A.shape[0]
+    wr, wc = (0, 0) if W is None else W.shape
+    need = 2 if (n < nb or nb == 0) else nb + 1
+    if wc < need or wr < n:
+        return None, MatopsError("Workspace too small")
+    if n == 0 or not flags & (LOWER | UPPER):
+        return A, None
+    if A.shape[1] != n:
+        raise ValueError("A must be square")
+    if len(ipiv) < n:
+        raise IndexError("index out of range")
+    Am, _, _ = _view(A)
+    check("mops_dsytrf_bk", _lib.lib().mops_dsytrf_bk(
+        ctypes.byref(Am), p, len(ipiv), flags, n, nb))
+    return A, None
+
+
+def SolveBK(B, A, ipiv, flags):
+    """B = A^-1 B from DecomposeBK's factors and pivots (ldlbks.go:303-316);
+    LOWER is tested first. UPPER follows LAPACK's dsytrs (the reference's
+    UPPER branch does not solve)."""
+    n = A.shape[0]
+    if n == 0 or B.shape[1] == 0:
+        return
+    piv = np.ascontiguousarray(ipiv, dtype=np.int64)
+    Bm, _, _ = _view(B)
+    Am, _, _ = _view(A)
+    check("mops_dsytrs_bk", _lib.lib().mops_dsytrs_bk(
+        ctypes.byref(Bm), ctypes.byref(Am), _pivot_array(piv), len(piv), flags, n,
+        B.shape[1]))
+
+
 def InverseTrm(A, flags, nb):
     """Inverse of the LOWER (else upper) triangle of A in place, UNIT for a
     unit diagonal (trinv.go:219-245). Returns (A, err)."""

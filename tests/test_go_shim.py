@@ -2,7 +2,8 @@
 here (no Go toolchain): every C function it calls is declared in
 include/mops.h and exported by libmops.so, and every function of the
 reference's Go API on this path has the same parameter and result types in
-the shim as in the reference (mmult.go, chol.go, lu.go, ldlnp.go, trinv.go;
+the shim as in the reference (mmult.go, chol.go, lu.go, ldlnp.go, trinv.go,
+ldl.go, ldlbk.go, ldlbks.go;
 parameter names may differ, e.g. C -> Cm since C names the cgo package).
 
 The reference's signatures are kept here as data (a list of type strings),
@@ -33,6 +34,12 @@ REFERENCE_SIGNATURES = {
                           ["*matrix.FloatMatrix", "error"]),
     "SolveLDLnoPiv": (["*matrix.FloatMatrix"] * 2 + ["Flags"], []),
     "InverseTrm": (["*matrix.FloatMatrix", "Flags", "int"], ["*matrix.FloatMatrix", "error"]),
+    "DecomposeLDL": (["*matrix.FloatMatrix"] * 2 + ["[]int", "Flags", "int"],
+                     ["*matrix.FloatMatrix", "error"]),
+    "SolveLDL": (["*matrix.FloatMatrix"] * 2 + ["[]int", "Flags"], []),
+    "DecomposeBK": (["*matrix.FloatMatrix"] * 2 + ["[]int", "Flags", "int"],
+                    ["*matrix.FloatMatrix", "error"]),
+    "SolveBK": (["*matrix.FloatMatrix"] * 2 + ["[]int", "Flags"], []),
     "BlockingParams": (["int"] * 3, []),
     "NumWorkers": (["int"], ["int"]),
     "SetPanicOnError": (["bool"], []),
@@ -91,5 +98,6 @@ def test_shim_error_strings_are_the_reference_ones():
     src = _go_sources()
     for msg in ("Mult: size mismatch", "MultSym: size mismatch", "A, B size mismatch",
                 "C not a square matrix", "A not a square matrix",
-                "pivot array < min(A.Rows(),A.Cols())", "No workspace for blocking invocation"):
+                "pivot array < min(A.Rows(),A.Cols())", "No workspace for blocking invocation",
+                "Workspace too small"):
         assert f'"{msg}"' in src, msg

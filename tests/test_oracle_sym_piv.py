@@ -21,7 +21,12 @@ the updated diagonal and for the last column of each block, so the pivot
 swap of that column is undone (findAndBuildPivot swaps wcol[0] and
 wcol[dmax], then the same two entries of the diagonal, which are the same
 memory) and the factorization is wrong; with nb + 1 columns it equals the
-unblocked one.
+unblocked one. And the reference's GEMV, calgo/dmvec_gemv.c:241, starts Y at
+R times X's increment (not Y's) on its unaligned path (odd leading
+dimension) once it splits more than MB = 68 rows into blocks: the blocked
+LOWER LDL's panel GEMV has a row of W as X, so for an odd order above 68
+the reference kernels write outside Y. Those shapes run on the restated
+kernels only (test_ldl_blocked_lower_odd_order_port).
 """
 import numpy as np
 import pytest
@@ -135,6 +140,20 @@ def test_ldl_blocked_lower_equals_unblocked_with_a_wider_workspace(kind, n, nb):
     assert O.DecomposeLDL(Au, None, ipu, LOWER, 0) is None
     np.testing.assert_array_equal(ipiv, ipu)
     assert np.abs(A - Au).max() <= 100 * n * EPS * np.abs(Au).max()
+
+
+@pytest.mark.parametrize("n,nb", [(129, 64), (257, 64), (201, 68)])
+def test_ldl_blocked_lower_odd_order_port(n, nb):
+    # restated kernels only: the reference's GEMV writes out of bounds here
+    O.use("port")
+    A0 = _sym(n, 31 + n)
+    A, Au = np.asfortranarray(A0.copy()), np.asfortranarray(A0.copy())
+    ipiv, ipu = np.zeros(n, np.int64), np.zeros(n, np.int64)
+    assert O.DecomposeLDL(A, np.zeros((n, nb + 1), order="F"), ipiv, LOWER, nb) is None
+    assert O.DecomposeLDL(Au, None, ipu, LOWER, 0) is None
+    np.testing.assert_array_equal(ipiv, ipu)
+    assert np.abs(A - Au).max() <= 100 * n * EPS * np.abs(Au).max()
+    assert _ldl_resid(A0, A, ipiv, True) <= 100 * n * EPS
 
 
 def test_reference_blocked_ldl_lower_defect_with_nb_columns(kind):
