@@ -51,11 +51,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other workloads (DGEMM 1024^3, LU and mixed LU n=16384, "
-                         "Cholesky adjoint n=4096)")
+                         "Cholesky adjoint n=4096, LDL and Bunch-Kaufman n=4096)")
     ap.add_argument("--profile-run", action="store_true",
                     help="one warm-up and one timed step plus a 256 MiB reference copy, for "
                          "rocprofv3 counter passes; prints no JSON")
-    ap.add_argument("--only", choices=["dgemm", "lu", "mixed", "adjoint"],
+    ap.add_argument("--only", choices=["dgemm", "lu", "mixed", "adjoint", "sympiv"],
                     help="run only this other workload (for profiling) and print its JSON")
     ap.add_argument("--kernel-stats", metavar="DB",
                     help="print the per-kernel duration table of a rocprofv3 --kernel-trace "
@@ -113,6 +113,8 @@ def main():
             lu, _ = extra_lu(L, ok, colmajor, torch, dev, sp, reps=0)
             res = extra_mixed(L, ok, colmajor, torch, dev, sp, lapack_solution(torch, dev, colmajor),
                               lu["total_ms"], reps=0)
+        elif args.only == "sympiv":
+            res = extra_sym_piv(L, ok, colmajor, torch, dev, sp)
         else:
             res = extra_adjoint(L, ok, colmajor, torch, dev, sp)
         print(json.dumps({args.only: res}), flush=True)
@@ -247,6 +249,7 @@ def main():
         out["mixed_lu_16384"] = extra_mixed(L, ok, colmajor, torch, dev, sp, xref,
                                             out["lu_16384"]["total_ms"])
         out["chol_adjoint_4096"] = extra_adjoint(L, ok, colmajor, torch, dev, sp)
+        out["ldl_bk_4096"] = extra_sym_piv(L, ok, colmajor, torch, dev, sp)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nb, n, nrhs)
@@ -438,6 +441,60 @@ def extra_adjoint(L, ok, colmajor, torch, dev, sp, n=4096):
             "symmetric": bool(torch.equal(Ab, Ab.T))}
 
 
+def extra_sym_piv(L, ok, colmajor, torch, dev, sp, n=4096, nrhs=16, reps=3):
+    """DecomposeLDL(A, W, ipiv, LOWER, 0) + SolveLDL and DecomposeBK(A, W, ipiv,
+    LOWER, 0) + SolveBK at n = 4096, 16 RHS, A = G + G^T with G ~ N(0,1)
+    (diagonal scaled by 0.05 for Bunch-Kaufman, which then takes 2x2 pivots):
+    best of `reps` device times (HIP events on the launch stream) of the
+    factorization and of the solve; TFLOP/s with n^3/3 flops (the
+    factorization) and 2 n^2 nrhs (the solve). Checked by the solve's
+    backward error."""
+    g = torch.Generator(device="cpu").manual_seed(9)
+    G = torch.randn((n, n), dtype=torch.float64, generator=g).to(dev)
+    S = G + G.T
+    del G
+    B0 = torch.randn((n, nrhs), dtype=torch.float64, generator=g).to(dev)
+    strm = torch.cuda.current_stream()  # the stream sp names
+    out = {"n": n, "nrhs": nrhs}
+    for name, scale, fac, sol in (("ldl", 1.0, "mops_d_dsytrf_ldl", "mops_d_dsytrs_ldl"),
+                                  ("bk", 0.05, "mops_d_dsytrf_bk", "mops_d_dsytrs_bk")):
+        A0 = S.clone()
+        A0.diagonal().mul_(scale)
+        A = colmajor(n, n)
+        X = colmajor(n, nrhs)
+        ipiv = torch.zeros(n, dtype=torch.int64, device=dev)
+        tf = ts = None
+        for _ in range(reps):
+            A.copy_(A0)
+            X.copy_(B0)
+            torch.cuda.synchronize()
+            e0, e1 = _events(torch)
+            e2 = torch.cuda.Event(enable_timing=True)
+            e0.record(strm)
+            if name == "ldl":
+                ok(getattr(L, fac)(LOWER, n, 0, A.data_ptr(), n, ipiv.data_ptr(), n, None, sp),
+                   fac)
+            else:
+                ok(getattr(L, fac)(LOWER, n, 0, A.data_ptr(), n, ipiv.data_ptr(), n, sp), fac)
+            e1.record(strm)
+            ok(getattr(L, sol)(LOWER, n, nrhs, A.data_ptr(), n, ipiv.data_ptr(), n,
+                               X.data_ptr(), n, sp), sol)
+            e2.record(strm)
+            torch.cuda.synchronize()
+            f, v = e0.elapsed_time(e1), e1.elapsed_time(e2)
+            tf = f if tf is None else min(tf, f)
+            ts = v if ts is None else min(ts, v)
+        R = A0 @ X - B0
+        eta = (R.abs().max() / (A0.abs().sum(dim=1).max() * X.abs().max() +
+                                B0.abs().max())).item()
+        out[name] = {"factor_ms": round(tf, 2), "solve_ms": round(ts, 3),
+                     "factor_tflops": round(n ** 3 / 3 / (tf * 1e-3) / 1e12, 3),
+                     "solve_tflops": round(2 * n * n * nrhs / (ts * 1e-3) / 1e12, 3),
+                     "two_by_two_blocks": int((ipiv < 0).sum().item()) // 2,
+                     "solve_backward_error": eta}
+    return out
+
+
 def kernel_stats(db):
     """Per-kernel calls / total / average / min / max duration (us) and share,
     from the kernels view of a rocprofv3 database."""
@@ -513,9 +570,10 @@ def traffic_summary(fetch_db, write_db):
 
 def cpu_baseline(nb, n, nrhs):
     """The CPU oracle (the reference's calgo kernels under the restated Go
-    glue) on the host: the Cholesky workload itself on one core (value), and
-    the DGEMM 1024^3 and a bounded LU sample (n = 4096, 16 RHS) with one and
-    with all host cores as Mult workers."""
+    glue) on the host: the Cholesky workload itself on one core (value), the
+    DGEMM 1024^3 and a bounded LU sample (n = 4096, 16 RHS) with one and
+    with all host cores as Mult workers, and LDL / Bunch-Kaufman with their
+    solves at n = 4096 on one core (the workload of the GPU leg)."""
     try:
         from oracle import oracle as orc
     except Exception as e:  # the baseline is reported, never required
@@ -525,6 +583,7 @@ def cpu_baseline(nb, n, nrhs):
     out["other_workloads"] = [orc.bench_mult_and_lu(1)]
     if ncores > 1:
         out["other_workloads"].append(orc.bench_mult_and_lu(ncores))
+    out["ldl_bk_4096"] = orc.bench_sym_piv_sample(n=4096)
     return out
 
 

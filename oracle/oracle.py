@@ -351,6 +351,40 @@ def bench_cholesky_sample(nb=256, n=8192, nrhs=64, seed=3):
     }
 
 
+def bench_sym_piv_sample(n=2048, nrhs=16, seed=9):
+    """One-core times of DecomposeLDL(A, W, ipiv, LOWER, 0) + SolveLDL and of
+    DecomposeBK(A, W, ipiv, LOWER, 0) + SolveBK (nrhs RHS) at order n, A =
+    G + G^T with G of N(0,1) entries (its diagonal scaled by 0.05 for
+    Bunch-Kaufman, which then takes 2x2 pivots), one run each after a cache
+    flush. TFLOP/s with n^3/3 + 2 n^2 nrhs flops."""
+    kind = "reference" if reference_available() else "port"
+    use(kind)
+    g = np.random.Generator(np.random.Philox(seed))
+    G = g.standard_normal((n, n))
+    S = G + G.T
+    B0 = np.asfortranarray(g.standard_normal((n, nrhs)))
+    flops = n ** 3 / 3.0 + 2.0 * n * n * nrhs
+    out = {"n": n, "nrhs": nrhs, "kind": kind, "cores": 1}
+    for name, scale, fac, sol in (("ldl", 1.0, DecomposeLDL, SolveLDL),
+                                  ("bk", 0.05, DecomposeBK, SolveBK)):
+        A = np.asfortranarray(S.copy())
+        A[np.diag_indices(n)] *= scale
+        X = B0.copy(order="F")
+        ipiv = np.zeros(n, np.int64)
+        W = np.zeros((n, 2), order="F")
+        msg = []
+
+        def run():
+            msg.append(fac(A, W, ipiv, LOWER, 0))
+            sol(X, A, ipiv, LOWER)
+
+        t = _timed(run)
+        if msg != [None]:
+            raise RuntimeError(f"oracle {name} failed on the sample: {msg}")
+        out[name] = {"seconds": round(t, 3), "tflops": round(flops / t / 1e12, 6)}
+    return out
+
+
 def bench_mult_and_lu(workers, n_mult=1024, n_lu=4096, nb=256, nrhs=16, mult_runs=5):
     """CPU legs of the other workloads with `workers` Mult workers
     (NumWorkers, mmult.go:69-73; the reference splits only Mult over them):
