@@ -484,23 +484,37 @@ __global__ void k_diag_right(int rows, int kb, double* X, long ldx, const double
 
 // perm = the row order after the interchanges rows k <-> p[k] - 1 for the
 // entries p[k] > 0 of p[0..npv), in order (forward) or last to first:
-// B'[i] = B[perm[i]]. Entries beyond n are ignored. One workgroup; the
-// sequence runs in LDS when it fits.
-constexpr int PERM_LDS = 32768;
+// B'[i] = B[perm[i]]. Entries beyond n are ignored. One workgroup: the
+// pivots are staged in LDS by all threads, then one thread runs the
+// sequence of interchanges on an index array in LDS (in global memory
+// beyond PERM_LDS rows).
+constexpr int PERM_LDS = 16384;
 __global__ __launch_bounds__(256) void k_pivots_to_perm(int n, int npv,
                                                         const int64_t* __restrict__ p,
                                                         int forward, int* perm) {
   __shared__ int q[PERM_LDS];
+  __shared__ int pv[PERM_LDS];
   const bool lds = n <= PERM_LDS;
   int* v = lds ? q : perm;
   for (int i = threadIdx.x; i < n; i += 256) v[i] = i;
+  if (lds)
+    for (int i = threadIdx.x; i < npv; i += 256) {
+      const long pk = p[i];
+      pv[i] = pk > 0 && pk <= n ? (int)pk - 1 : -1;
+    }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int s = 0; s < npv; ++s) {
       const int k = forward ? s : npv - 1 - s;
-      const long pk = p[k];
-      if (pk > 0 && pk <= n) {
-        const int o = (int)pk - 1, t = v[k];
+      int o;
+      if (lds) {
+        o = pv[k];
+      } else {
+        const long pk = p[k];
+        o = pk > 0 && pk <= n ? (int)pk - 1 : -1;
+      }
+      if (o >= 0) {
+        const int t = v[k];
         v[k] = v[o];
         v[o] = t;
       }
@@ -682,6 +696,211 @@ __global__ __launch_bounds__(256) void k_bk_solve(int n, const double* __restric
   }
 }
 
+// SolveBK with the right-hand side and the pivots in LDS, for n <= BKS_MAX:
+// one workgroup of BKS_T threads per right-hand side column, thread t owns
+// rows t + BKS_T m (m < BKS_R); the L column(s) of the next block are
+// loaded into registers while the current block is applied, so the
+// sequential sweep waits on LDS and barriers instead of global memory.
+// Same order of operations per block as k_bk_solve.
+constexpr int BKS_T = 1024, BKS_MAX = 8 * BKS_T;
+
+template <int R>
+__device__ __forceinline__ void bks_load(const double* __restrict__ A, long lda, int col, bool two,
+                                         int lo, int hi, double (&c0)[R], double (&c1)[R]) {
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int i = threadIdx.x + BKS_T * m;
+    const bool in = i >= lo && i < hi;
+    c0[m] = in ? A[i + (long)col * lda] : 0.0;
+    c1[m] = in && two ? A[i + (long)(col + 1) * lda] : 0.0;
+  }
+}
+
+// the sum over the workgroup of v, in every thread (red: 16 doubles)
+__device__ __forceinline__ double bks_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < BKS_T / 64; ++w) r += red[w];
+  return r;
+}
+
+template <bool UPPER, int R>
+__global__ __launch_bounds__(BKS_T) void k_bk_solve_lds(int n, const double* __restrict__ A,
+                                                        long lda, const int64_t* __restrict__ p,
+                                                        double* B, long ldb) {
+  __shared__ double lb[BKS_MAX];
+  __shared__ int lp[BKS_MAX];
+  __shared__ double red[2][BKS_T / 64];
+  double* b = B + (long)blockIdx.x * ldb;
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += BKS_T) {
+    lb[i] = b[i];
+    const long v = p[i];
+    lp[i] = v >= -n && v <= n ? (int)v : 0;  // 0: no interchange, a 1x1 block
+  }
+  __syncthreads();
+  // the 2 x 2 block solve of rows k, k + 1 (ldlbks.go:62-80), by thread 0
+  auto dsolve = [&](int k, int np) {
+    if (np == 1) {
+      lb[k] = lb[k] / at(A, lda, k, k);
+    } else {
+      double x0 = lb[k], x1 = lb[k + 1];
+      solve_e(at(A, lda, k, k), UPPER ? at(A, lda, k, k + 1) : at(A, lda, k + 1, k),
+              at(A, lda, k + 1, k + 1), &x0, &x1);
+      lb[k] = x0;
+      lb[k + 1] = x1;
+    }
+  };
+  auto swp = [&](int i, int j) {
+    if (j >= 0 && j < n) {
+      const double v = lb[i];
+      lb[i] = lb[j];
+      lb[j] = v;
+    }
+  };
+  double c0[R], c1[R], x0[R], x1[R];
+  // ---- phase 1: interchange, rows beyond the block -= its columns times
+  // b1, b1 = E^-1 b1; blocks first to last (LOWER) or last to first
+  {
+    int k, np;
+    if (!UPPER) {
+      k = 0;
+      np = lp[0] < 0 && n > 1 ? 2 : 1;
+    } else {
+      np = lp[n - 1] < 0 && n > 1 ? 2 : 1;
+      k = n - np;
+    }
+    bks_load(A, lda, k, np == 2, UPPER ? 0 : k + np, UPPER ? k : n, c0, c1);
+    int pk = -1, pnp = 0;  // the block whose E^-1 is pending
+    while (true) {
+      int nk, nnp;  // the next block
+      if (!UPPER) {
+        nk = k + np;
+        nnp = nk < n ? (lp[nk] < 0 && nk + 1 < n ? 2 : 1) : 0;
+      } else {
+        nnp = k > 0 ? (lp[k - 1] < 0 && k > 1 ? 2 : 1) : 0;
+        nk = k - nnp;
+      }
+      if (t == 0) {
+        if (pnp) dsolve(pk, pnp);
+        const int v = lp[k];
+        if (!UPPER) {
+          if (np == 1) {
+            if (v != k + 1) swp(k, v - 1);
+          } else if (-v != k + 2) {
+            swp(k + 1, -v - 1);
+          }
+        } else {
+          const int kk = k + np;  // 1-based last row of the block
+          if (np == 1) {
+            if (v != kk) swp(k, v - 1);
+          } else if (-v != kk - 1) {
+            swp(k, -v - 1);
+          }
+        }
+      }
+      __syncthreads();
+      const double b0 = lb[k], b1 = np == 2 ? lb[k + 1] : 0.0;
+      if (nnp) bks_load(A, lda, nk, nnp == 2, UPPER ? 0 : nk + nnp, UPPER ? nk : n, x0, x1);
+      const int lo = UPPER ? 0 : k + np, hi = UPPER ? k : n;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = t + BKS_T * m;
+        if (i >= lo && i < hi) {
+          double v = lb[i] - c0[m] * b0;
+          if (np == 2) v -= c1[m] * b1;
+          lb[i] = v;
+        }
+      }
+      __syncthreads();
+      pk = k;
+      pnp = np;
+      if (!nnp) break;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        c0[m] = x0[m];
+        c1[m] = x1[m];
+      }
+      k = nk;
+      np = nnp;
+    }
+    if (t == 0 && pnp) dsolve(pk, pnp);
+    __syncthreads();
+  }
+  // ---- phase 2: b1 -= the block's columns^T times the rows beyond it,
+  // then the interchange; blocks last to first (LOWER) or first to last
+  {
+    int k, np;
+    if (!UPPER) {
+      np = lp[n - 1] < 0 && n > 1 ? 2 : 1;
+      k = n - np;
+    } else {
+      k = 0;
+      np = lp[0] < 0 && n > 1 ? 2 : 1;
+    }
+    bks_load(A, lda, k, np == 2, UPPER ? 0 : k + np, UPPER ? k : n, c0, c1);
+    int par = 0;
+    while (true) {
+      int nk, nnp;
+      if (!UPPER) {
+        nnp = k > 0 ? (lp[k - 1] < 0 && k > 1 ? 2 : 1) : 0;
+        nk = k - nnp;
+      } else {
+        nk = k + np;
+        nnp = nk < n ? (lp[nk] < 0 && nk + 1 < n ? 2 : 1) : 0;
+      }
+      if (nnp) bks_load(A, lda, nk, nnp == 2, UPPER ? 0 : nk + nnp, UPPER ? nk : n, x0, x1);
+      const int lo = UPPER ? 0 : k + np, hi = UPPER ? k : n;
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = t + BKS_T * m;
+        if (i >= lo && i < hi) {
+          const double v = lb[i];
+          s0 += c0[m] * v;
+          s1 += c1[m] * v;
+        }
+      }
+      s0 = bks_sum(s0, red[par]);
+      if (np == 2) s1 = bks_sum(s1, red[par ^ 1]);
+      if (t == 0) {
+        lb[k] -= s0;
+        if (np == 2) lb[k + 1] -= s1;
+        const int v = lp[k];
+        if (!UPPER) {
+          const int nc = k + np;
+          if (np == 1) {
+            if (v != nc) swp(k, v - 1);
+          } else if (-v != nc) {
+            swp(k + 1, -v - 1);
+          }
+        } else {
+          if (np == 1) {
+            if (v != k + 1) swp(k, v - 1);
+          } else if (-v != k + 1) {
+            swp(k, -v - 1);
+          }
+        }
+      }
+      __syncthreads();
+      par ^= 1;
+      if (!nnp) break;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        c0[m] = x0[m];
+        c1[m] = x1[m];
+      }
+      k = nk;
+      np = nnp;
+    }
+  }
+  for (int i = t; i < n; i += BKS_T) b[i] = lb[i];
+}
+
 }  // namespace
 
 size_t ldl_piv_ws_bytes(int n, int nb, bool upper) {
@@ -828,10 +1047,25 @@ int ldl_piv_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A,
 int bk_solve_d(hipStream_t s, bool upper, int n, int nrhs, const double* A, long lda,
                const int64_t* ipiv, double* B, long ldb) {
   if (n <= 0 || nrhs <= 0) return ST_OK;
-  if (upper)
+  if (n <= 4 * BKS_T) {
+    if (upper)
+      hipLaunchKernelGGL((k_bk_solve_lds<true, 4>), dim3(nrhs), dim3(BKS_T), 0, s, n, A, lda, ipiv,
+                         B, ldb);
+    else
+      hipLaunchKernelGGL((k_bk_solve_lds<false, 4>), dim3(nrhs), dim3(BKS_T), 0, s, n, A, lda, ipiv,
+                         B, ldb);
+  } else if (n <= BKS_MAX) {
+    if (upper)
+      hipLaunchKernelGGL((k_bk_solve_lds<true, 8>), dim3(nrhs), dim3(BKS_T), 0, s, n, A, lda, ipiv,
+                         B, ldb);
+    else
+      hipLaunchKernelGGL((k_bk_solve_lds<false, 8>), dim3(nrhs), dim3(BKS_T), 0, s, n, A, lda, ipiv,
+                         B, ldb);
+  } else if (upper) {
     hipLaunchKernelGGL(k_bk_solve<true>, dim3(nrhs), dim3(256), 0, s, n, A, lda, ipiv, B, ldb);
-  else
+  } else {
     hipLaunchKernelGGL(k_bk_solve<false>, dim3(nrhs), dim3(256), 0, s, n, A, lda, ipiv, B, ldb);
+  }
   MOPS_CHECK_HIP(hipGetLastError());
   return ST_OK;
 }

@@ -333,6 +333,22 @@ def test_solve_bk(uplo, n, nrhs):
         assert np.abs(X - Xo).max() <= tol
 
 
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+@pytest.mark.parametrize("n", [4097, 8192, 8300])
+def test_solve_bk_large_orders(uplo, n):
+    # the solve keeps the right-hand side in LDS up to n = 8192 (4 rows per
+    # thread to 4096, 8 beyond) and sweeps in global memory above
+    A0 = _sym(n, 90 + n, 0.05)
+    A = A0.copy(order="F")
+    ipiv = np.zeros(n, np.int64)
+    mo.DecomposeBK(A, np.zeros((n, 2), order="F"), ipiv, uplo, 0)
+    assert (ipiv < 0).any()
+    B0 = np.asfortranarray(np.random.default_rng(n).standard_normal((n, 3)))
+    X = B0.copy(order="F")
+    mo.SolveBK(X, A, ipiv, uplo)
+    assert _eta(A0, X, B0) <= 10 * n * EPS
+
+
 # ------------------------------------------------- full size, device twins
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
 def test_ldl_and_bk_4096_reconstruct_and_solve(uplo):
