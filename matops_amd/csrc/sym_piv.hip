@@ -51,6 +51,7 @@ struct SymStep {
   int np;    // 1 or 2 columns
   int skip;  // this step does nothing (past the end, or stopped)
   int err;   // 0, or 1 + the index of a zero pivot (LDL)
+  double d;  // blocked LDL: the pivot of step k (its updated diagonal)
 };
 
 constexpr double BK_ALPHA = 0.6403882032022075;  // (1 + sqrt(17)) / 8, ldlbk.go:16
@@ -442,6 +443,309 @@ __global__ __launch_bounds__(256) void k_ldl_tail_fixup(const SymStep* st, doubl
   for (int q = c0 + threadIdx.x; q <= e; q += 256)
     if (ipiv[q] > 0) ipiv[q] -= c0;
 }
+
+// ---- blocked LOWER LDL: panels of PW columns with delayed updates ----
+// (blkLowerLDL with its bounded panel, ldl.go:90-295, with the workspace of
+// PW + 1 columns that keeps the updated diagonal apart.) Per step k of a
+// panel starting at k0 (j = k - k0):
+//   column  a = A[k+1:, k] - L[k+1:, k0:k] W[k, 0:j]^T (the updated column;
+//           findAndBuildPivot's GEMV), W[:, j] = a, L[:, k] = a / d_k,
+//           diag -= a a / d_k (MVUpdateDiag), d_k = diag[k] stored in A[k][k]
+//   decide  IAMax of the updated diagonal from k + 1 (each workgroup offers
+//           the winner of its rows, the last one to finish reduces them)
+//   swap    the symmetric interchange of the next step in the left columns,
+//           the (not yet updated) trailing part, diag and W's rows
+// so one launch per step; after the panel, A22 -= L21 W21^T as one
+// triangle-only GEMM. The pivots are those of the unblocked algorithm on
+// the same values; the updated values differ from its rank-1 updates by
+// rounding only.
+constexpr int PW = 64;     // panel width
+constexpr int LC_T = 256;  // rows per workgroup of the column kernels
+
+__device__ __forceinline__ double ld_ag(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The reference's IAMax offer of this thread's row: rows e = 0, 1, ... of
+// the search pair up as (2q, 2q + 1) on neighbouring lanes; the pair offers
+// its larger |value| (the first on a tie; a NaN first hides the second), a
+// lone last row its own; an offer counts only when above zero (the running
+// maximum starts at 0.0). in: this thread's row is in the search.
+__device__ __forceinline__ void pair_offer(double v, bool in, int e, double& ov, int& oi) {
+  const double c = in ? fabs(v) : 0.0;
+  const double cp = __shfl_xor(c, 1, 64);
+  const bool pin = __shfl_xor((int)in, 1, 64) != 0;
+  ov = 0.0;
+  oi = 0x7fffffff;
+  if ((e & 1) == 0 && in) {
+    double b = c;
+    int bi = e;
+    if (pin && cp > c) {
+      b = cp;
+      bi = e + 1;
+    }
+    if (b > 0.0) {
+      ov = b;
+      oi = bi;
+    }
+  }
+}
+
+// max |value| over the workgroup, ties to the lower index (every thread
+// gets it); NaN offers never win
+__device__ __forceinline__ void wg_best(double& v, int& i, double* sv, int* si) {
+  const int t = threadIdx.x;
+  sv[t] = v;
+  si[t] = i;
+  __syncthreads();
+  for (int h = LC_T / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      const double w = sv[t + h];
+      const int wi = si[t + h];
+      if (w > sv[t] || (w == sv[t] && wi < si[t])) {
+        sv[t] = w;
+        si[t] = wi;
+      }
+    }
+    __syncthreads();
+  }
+  v = sv[0];
+  i = si[0];
+  __syncthreads();
+}
+
+// Shared tail of the panel kernels, run by the workgroup that finished last:
+// reduce the G partial offers, record the pivot of step kd (ipiv, st->k,
+// st->r, st->d, the zero check) and make the part of its interchange that
+// the next column kernel does not: rows kd and p of the panel's L columns
+// k0..kd-1 and of W's first jw columns, the two diagonal entries of A and
+// of diag. The trailing column / row moves are made by the next column
+// kernel, the rows of the columns left of the panel after the panel.
+__device__ void ldl_decide(int n, double* A, long lda, double* diag, double* W, long ldw, int k0,
+                           int jw, SymStep* st, int64_t* ipiv, const double* pv, const int* pi,
+                           int G, int kd, int check_from, double* sv, int* si) {
+  const int t = threadIdx.x;
+  double v = 0.0;
+  int vi = 0x7fffffff;
+  for (int b = t; b < G; b += LC_T) {
+    const double w = ld_ag(pv + b);
+    const int wi = __hip_atomic_load(pi + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w > v || (w == v && wi < vi)) {
+      v = w;
+      vi = wi;
+    }
+  }
+  wg_best(v, vi, sv, si);
+  const int r = vi == 0x7fffffff ? 0 : vi;
+  const int p = kd + r;
+  if (t == 0) {
+    const double dp = ld_ag(diag + p);
+    ipiv[kd] = r > 0 ? p + 1 : 0;
+    st->k = kd;
+    st->r = r;
+    st->d = dp;
+    if (kd >= check_from && dp == 0.0) st->err = 1 + kd;
+  }
+  if (r == 0) return;
+  __syncthreads();  // diag[p] is read before it is swapped
+  const int nl = kd - k0;  // panel columns left of kd
+  for (int e = t; e < nl + jw + 2; e += LC_T) {
+    double *x, *y;
+    if (e < nl) {
+      x = &at(A, lda, kd, k0 + e);
+      y = &at(A, lda, p, k0 + e);
+    } else if (e < nl + jw) {
+      x = W + kd + (long)(e - nl) * ldw;
+      y = W + p + (long)(e - nl) * ldw;
+    } else if (e == nl + jw) {
+      x = diag + kd;
+      y = diag + p;
+    } else {
+      x = &at(A, lda, kd, kd);
+      y = &at(A, lda, p, p);
+    }
+    const double a = ld_ag(x), c = ld_ag(y);
+    *x = c;
+    *y = a;
+  }
+}
+
+// last-workgroup election: every workgroup's agent-scope stores are
+// complete before it arrives
+__device__ __forceinline__ bool arrive_last(unsigned* ctr, unsigned base, int* flag) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    *flag = prev == base + gridDim.x - 1;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+// Panel start k0: diag = the trailing diagonal, then the decision of step k0.
+__global__ __launch_bounds__(LC_T) void k_ldl_pinit(int n, double* A, long lda, int k0,
+                                                    double* diag, double* W, long ldw,
+                                                    SymStep* st, int64_t* ipiv, double* pv,
+                                                    int* pi, unsigned* ctr, unsigned base,
+                                                    int check_from) {
+  __shared__ double sv[LC_T];
+  __shared__ int si[LC_T];
+  __shared__ int last;
+  const int t = threadIdx.x, b = blockIdx.x;
+  const bool halt = st->err != 0;
+  const int e = b * LC_T + t, i = k0 + e;
+  const bool in = !halt && i < n;
+  double d = 0.0;
+  if (in) {
+    d = at(A, lda, i, i);
+    st_ag(diag + i, d);
+  }
+  double ov;
+  int oi;
+  pair_offer(d, in, e, ov, oi);
+  wg_best(ov, oi, sv, si);
+  if (t == 0) {
+    st_ag(pv + b, ov);
+    __hip_atomic_store(pi + b, oi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (!arrive_last(ctr, base, &last) || halt) return;
+  ldl_decide(n, A, lda, diag, W, ldw, k0, 0, st, ipiv, pv, pi, gridDim.x, k0, check_from, sv, si);
+}
+
+// Step k = k0 + j of the panel (pivot st->r, st->d decided before): the
+// trailing moves of its interchange fused with the column (row i's new
+// column value is read where the interchange puts it, the value it
+// displaces is moved there), then (decide) the decision of step k + 1.
+__global__ __launch_bounds__(LC_T) void k_ldl_pcol(int n, double* A, long lda, int k0, int j,
+                                                   double* diag, double* W, long ldw,
+                                                   SymStep* st, int64_t* ipiv, double* pv,
+                                                   int* pi, unsigned* ctr, unsigned base,
+                                                   int check_from, int decide) {
+  __shared__ double sv[LC_T];
+  __shared__ int si[LC_T];
+  __shared__ int last;
+  const int t = threadIdx.x, b = blockIdx.x;
+  const int k = k0 + j;
+  const bool halt = st->err != 0;
+  const int e = b * LC_T + t, i = k + 1 + e;
+  const bool in = !halt && i < n;
+  double nd = 0.0;
+  if (!halt) {
+    const double dk = st->d;
+    const int p = k + st->r;
+    if (b == 0 && t == 0) at(A, lda, k, k) = dk;
+    if (in) {
+      double a = at(A, lda, i, k);
+      if (p != k) {  // applyPivotSym LOWER, pivot.go:167-172
+        if (i < p) {
+          const double x = at(A, lda, p, i);
+          at(A, lda, p, i) = a;
+          a = x;
+        } else if (i > p) {
+          const double x = at(A, lda, i, p);
+          at(A, lda, i, p) = a;
+          a = x;
+        }
+      }
+      const double* Li = A + i + (long)k0 * lda;
+      const double* Wk = W + k;
+#pragma unroll 8
+      for (int q = 0; q < j; ++q) a -= Li[(long)q * lda] * Wk[(long)q * ldw];
+      st_ag(W + i + (long)j * ldw, a);
+      st_ag(&at(A, lda, i, k), a / dk);
+      nd = diag[i] + a * a * (-1.0 / dk);
+      st_ag(diag + i, nd);
+    }
+  }
+  double ov;
+  int oi;
+  pair_offer(nd, in, e, ov, oi);
+  wg_best(ov, oi, sv, si);
+  if (t == 0) {
+    st_ag(pv + b, ov);
+    __hip_atomic_store(pi + b, oi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (!arrive_last(ctr, base, &last) || halt || !decide || k + 1 >= n) return;
+  ldl_decide(n, A, lda, diag, W, ldw, k0, j + 1, st, ipiv, pv, pi, gridDim.x, k + 1, check_from,
+             sv, si);
+}
+
+// The trailing moves of step st->k's interchange (for a step whose column
+// kernel does not run: the one with a zero pivot).
+__global__ void k_ldl_swap_trailing(int n, double* A, long lda, const SymStep* st) {
+  const int kd = st->k, r = st->r, p = kd + r;
+  if (r == 0) return;
+  for (int i = kd + 1 + blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    if (i < p)
+      swap_el(&at(A, lda, i, kd), &at(A, lda, p, i));
+    else if (i > p)
+      swap_el(&at(A, lda, i, kd), &at(A, lda, i, p));
+  }
+}
+
+// The interchanges of steps k0 .. k0 + w - 1 (ipiv, 1-based, 0 = none, in
+// order) on the rows of columns 0 .. ncol - 1 (the columns left of the
+// panel; applyRowPivots FORWARD, ldl.go:288). Each workgroup composes the
+// row moves in LDS (rows k0 .. k0 + w - 1 and the distinct pivot rows
+// beyond), then moves the affected rows of its RP_C columns.
+constexpr int RP_C = 32;  // columns per workgroup of k_ldl_panel_rows
+__global__ __launch_bounds__(256) void k_ldl_panel_rows(int ncol, double* A, long lda, int k0,
+                                                        int w, const int64_t* __restrict__ ipiv,
+                                                        int n) {
+  __shared__ double vals[2 * PW * RP_C];
+  __shared__ int row[2 * PW];  // row of slot s
+  __shared__ int src[2 * PW];  // the row whose value ends in row[s]
+  __shared__ int ns;
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < w; ++s) {
+      row[s] = k0 + s;
+      src[s] = k0 + s;
+    }
+    int m = w;
+    for (int j = 0; j < w; ++j) {
+      const long pk = ipiv[k0 + j];
+      if (pk <= 0 || pk > n) continue;
+      const int pr = (int)pk - 1;
+      int sp = -1;
+      if (pr < k0 + w) {
+        sp = pr - k0;
+      } else {
+        for (int q = w; q < m; ++q)
+          if (row[q] == pr) sp = q;
+        if (sp < 0) {
+          sp = m++;
+          row[sp] = pr;
+          src[sp] = pr;
+        }
+      }
+      const int x = src[j];
+      src[j] = src[sp];
+      src[sp] = x;
+    }
+    ns = m;
+  }
+  __syncthreads();
+  // all sources of this workgroup's RP_C columns are read into LDS before
+  // any destination is written (the moves permute the slot rows)
+  const int m = ns, c0 = blockIdx.x * RP_C;
+  for (int e = threadIdx.x; e < m * RP_C; e += 256) {
+    const int q = e / RP_C, c = c0 + e % RP_C;
+    if (c < ncol) vals[e] = A[src[q] + (long)c * lda];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m * RP_C; e += 256) {
+    const int q = e / RP_C, c = c0 + e % RP_C;
+    if (c < ncol) A[row[q] + (long)c * lda] = vals[e];
+  }
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 // The interchanges of a block's pivots p[0..kb) (1-based within the block,
 // 0 = none), last to first (BACKWARD, pivot.go:63-97), on the columns
@@ -903,9 +1207,86 @@ __global__ __launch_bounds__(BKS_T) void k_bk_solve_lds(int n, const double* __r
 
 }  // namespace
 
+namespace {
+
+// The blocked LOWER LDL on the n x n matrix A (state st initialised, ipiv
+// cleared): panels of PW columns; zero pivots are checked from check_from
+// on. After a panel with checks the host reads the state: on a zero pivot
+// at step e the pending updates of the panel's earlier steps are applied
+// to A[e:, e:] (the unblocked algorithm's state at its error) and the
+// panels stop.
+int ldl_lower_panels(hipStream_t s, int n, double* A, long lda, int64_t* ipiv, SymStep* st,
+                     int check_from) {
+  const int G = ceil_div(n, LC_T);
+  double* diag = (double*)ws_carve((size_t)n * sizeof(double));
+  double* W = (double*)ws_carve((size_t)n * PW * sizeof(double));
+  double* pv = (double*)ws_carve((size_t)G * sizeof(double));
+  int* pi = (int*)ws_carve((size_t)G * sizeof(int));
+  unsigned* ctr = (unsigned*)ws_carve(sizeof(unsigned));
+  if (!diag || !W || !pv || !pi || !ctr) {
+    set_error("ldl: workspace too small");
+    return ST_NOMEM;
+  }
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ctr);
+  unsigned base = 0;
+  for (int k0 = 0; k0 < n; k0 += PW) {
+    const int w = min(PW, n - k0);
+    int g = ceil_div(n - k0, LC_T);
+    hipLaunchKernelGGL(k_ldl_pinit, dim3(g), dim3(LC_T), 0, s, n, A, lda, k0, diag, W, (long)n,
+                       st, ipiv, pv, pi, ctr, base, check_from);
+    base += g;
+    for (int j = 0; j < w; ++j) {
+      const int rows = n - (k0 + j) - 1;
+      g = rows > 0 ? ceil_div(rows, LC_T) : 1;
+      hipLaunchKernelGGL(k_ldl_pcol, dim3(g), dim3(LC_T), 0, s, n, A, lda, k0, j, diag, W,
+                         (long)n, st, ipiv, pv, pi, ctr, base, check_from, j + 1 < w ? 1 : 0);
+      base += g;
+    }
+    MOPS_CHECK_HIP(hipGetLastError());
+    if (k0 + w - 1 >= check_from) {
+      int err = 0;
+      MOPS_CHECK_HIP(hipMemcpyAsync(&err, &st->err, sizeof(int), hipMemcpyDeviceToHost, s));
+      MOPS_CHECK_HIP(hipStreamSynchronize(s));
+      if (err) {
+        // complete the interchange of the zero-pivot step e, move the rows
+        // of the columns left of the panel for steps k0 .. e, and apply the
+        // pending updates of steps k0 .. e - 1 to A[e:, e:]
+        const int e = err - 1, kp = e - k0;
+        hipLaunchKernelGGL(k_ldl_swap_trailing, dim3(grid_for(n - e)), dim3(256), 0, s, n, A, lda,
+                           st);
+        if (k0 > 0)
+          hipLaunchKernelGGL(k_ldl_panel_rows, dim3(ceil_div(k0, RP_C)), dim3(256), 0, s, k0, A,
+                             lda, k0, kp + 1, ipiv, n);
+        MOPS_CHECK_HIP(hipGetLastError());
+        if (kp > 0 && e < n)
+          MOPS_TRY(gemm_d(s, false, true, n - e, n - e, kp, -1.0, A + e + (long)k0 * lda, lda,
+                          W + e, (long)n, 1.0, A + e + (long)e * lda, lda, GM_LOWER));
+        return ST_OK;
+      }
+    }
+    if (k0 > 0) {
+      hipLaunchKernelGGL(k_ldl_panel_rows, dim3(ceil_div(k0, RP_C)), dim3(256), 0, s, k0, A, lda,
+                         k0, w, ipiv, n);
+      MOPS_CHECK_HIP(hipGetLastError());
+    }
+    if (k0 + w < n) {
+      const int m2 = n - k0 - w;
+      MOPS_TRY(gemm_d(s, false, true, m2, m2, w, -1.0, A + (k0 + w) + (long)k0 * lda, lda,
+                      W + (k0 + w), (long)n, 1.0, A + (k0 + w) + (long)(k0 + w) * lda, lda,
+                      GM_LOWER));
+    }
+  }
+  return ST_OK;
+}
+
+}  // namespace
+
 size_t ldl_piv_ws_bytes(int n, int nb, bool upper) {
   if (n <= 0) return 0;
   size_t b = 1024;
+  if (!upper)
+    b += (size_t)n * (PW + 1) * sizeof(double) + (size_t)ceil_div(n, LC_T) * 12 + 2048 +
+         gemm_split_ws_bytes(n, n, PW, sizeof(double));
   if (upper && nb > 0 && n >= nb)
     b += (size_t)n * nb * sizeof(double) + trsm_ws_bytes(F_RIGHT, nb, n) +
          gemm_split_ws_bytes(n, n, nb, sizeof(double)) + 1024;
@@ -937,7 +1318,7 @@ int ldl_piv_d(hipStream_t s, bool upper, int n, int nb, double* A, long lda, int
       check_from = k;
     }
     hipLaunchKernelGGL(k_sym_init, dim3(1), dim3(1), 0, s, st, 0);
-    MOPS_TRY((sym_steps<false, false>(s, n, A, lda, ipiv, st, 0, check_from)));
+    MOPS_TRY(ldl_lower_panels(s, n, A, lda, ipiv, st, check_from));
     if (check_from > 0) {
       hipLaunchKernelGGL(k_ldl_tail_fixup, dim3(1), dim3(256), 0, s, st, A, lda, ipiv, check_from);
       MOPS_CHECK_HIP(hipGetLastError());
