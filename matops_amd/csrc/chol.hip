@@ -7,7 +7,7 @@
 //   A21 = A21 * tril(A11)^-T          (SolveTrm RIGHT|LOWER|TRANSA, :79)
 //   A22 = A22 - A21 * A21^T           (RankUpdateSym LOWER, :81)
 // and the UPPER variant with A12 / U11^-T (:82-87). The same step runs here
-// with the panel (diagonal block and the rows below it, up to 256 columns)
+// with the panel (diagonal block and the rows below it, up to 512 columns)
 // factored in one launch of k_chol_panel: 64-column diagonal blocks in
 // registers (the reference's column algorithm), their rows below solved and
 // the rest of the panel updated on the matrix cores, the workgroups chained
@@ -71,9 +71,10 @@ __device__ __forceinline__ void cp_wait(const int* f, int* err) {
   }
 }
 
-constexpr int CP_FLAGS = 4 + 16;  // ints per panel launch
+constexpr int CP_MAXB = 8;  // diagonal blocks of 64 in one panel launch (512 columns)
+constexpr int CP_FLAGS = CP_MAXB + CP_MAXB * CP_MAXB;  // ints per panel launch
 
-// One Cholesky panel of up to 256 columns (the diagonal blocks of 64 and
+// One Cholesky panel of up to 512 columns (the diagonal blocks of 64 and
 // every row below them) in one launch of 256-thread workgroups.
 //
 // Rows are split into 64-row chunks: workgroup b < nsteps owns the rows of
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
   const int nr = diag ? min(LEAF, pw - row0) : min(LEAF, m - row0);  // rows held
   const int jend = diag ? b : nsteps;                                 // solves
   int* dflag = flags;      // dflag[j]: diagonal block j factored, inverses published
-  int* xflag = flags + 4;  // xflag[4 j + c]: rows of block c solved in step j
+  int* xflag = flags + CP_MAXB;  // xflag[CP_MAXB j + c]: rows of block c solved in step j
   double* Bs = s1;         // Bs[r * CL_LD + k]: right-hand side, then X, of step j
   double* Ls = s2;         // L below the 16 x 16 diagonal blocks, or a published X
 
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
     if (diag) {
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      if (t == 0) __hip_atomic_store(&xflag[4 * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == 0) __hip_atomic_store(&xflag[CP_MAXB * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // updates of the column blocks right of j, right to left, so that the
     // block of column j + 1 (the next right-hand side) comes last
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
         }
         continue;
       }
-      if (t == 0) cp_wait(&xflag[4 * j + cb], err);
+      if (t == 0) cp_wait(&xflag[CP_MAXB * j + cb], err);
       __syncthreads();
       {
         double v[LEAF * LEAF / CL_T];  // all loads in flight before the first LDS write
@@ -464,14 +465,14 @@ __global__ void k_fix_info(int* info, const int* err) {
 // CL_FUSE columns: one launch of k_chol_panel. Wider panels: the recursive
 // (left part, trapezoidal update of the right part, right part) scheme. The
 // UPPER case is the transpose of each step.
-constexpr int CL_FUSE = 4 * LEAF;
+constexpr int CL_FUSE = CP_MAXB * LEAF;
 
 struct LeafFlags {
   int* flags;    // CP_FLAGS zeroed flags per panel launch
   int next;
   int cap;
   int* err;
-  double* minv;  // inverses of the current panel's 16 x 16 diagonal blocks (4 x 1024)
+  double* minv;  // inverses of the current panel's 16 x 16 diagonal blocks (CP_MAXB x 1024)
 };
 
 int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long lda, LeafFlags& fl,
@@ -510,16 +511,31 @@ int chol_rec(hipStream_t s, bool upper, int N, int j0, int w, double* A, long ld
 
 }  // namespace
 
-// Panel launches of one factorization: each block column of nb contributes
-// ceil(kb / 256) launches, and its recursive split at most one more.
-static int potrf_panel_count(int N, int nb) {
+// The outer block the GPU uses for a requested nb. DecomposeCHOL runs
+// unblocked when N < nb or nb == 0 (chol.go:121); the unblocked and blocked
+// algorithms compute the same factor, so both run here as the blocked one
+// with the block clamped to N. Blocks of 256 and more take 512 columns: a
+// panel of 512 is one launch of k_chol_panel, and the trailing updates then
+// have k = 512 (the triangle-only GEMM runs 38 TFLOP/s at k = 256 and 49 at
+// k = 512 for n = 7680; the Cholesky n = 8192 7.14 -> 6.61 ms). The factor
+// differs from the nb = 256 one by rounding only.
+static int potrf_block(int N, int nb) {
   if (nb <= 0 || nb > N) nb = N;
+  if (nb >= 4 * LEAF) nb = min(N, CL_FUSE);
+  return nb;
+}
+
+// Panel launches of one factorization: each block column of nb contributes
+// at most ceil(kb / 256) launches (ceil(kb / CL_FUSE) when it is split), and
+// its recursive split at most one more.
+static int potrf_panel_count(int N, int nb) {
+  nb = potrf_block(N, nb);
   return ceil_div(N, 4 * LEAF) + 2 * ceil_div(N, nb > 0 ? nb : 1) + 2;
 }
 
 size_t potrf_ws_bytes(int N, int nb) {
   return (size_t)(CP_FLAGS * potrf_panel_count(N, nb) + 8) * sizeof(int) +
-         (size_t)4 * 1024 * sizeof(double) + 2048;
+         (size_t)CP_MAXB * 1024 * sizeof(double) + 2048;
 }
 
 namespace {
@@ -560,15 +576,12 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
   // blockedCHOL takes the LOWER branch when the bit is set, otherwise UPPER
   // (chol.go:40, :77).
   const bool upper = !(flags & F_LOWER);
-  // DecomposeCHOL: unblocked when N < nb or nb == 0 (chol.go:121). The
-  // unblocked and blocked algorithms compute the same factor; here both are
-  // the blocked one with the outer block clamped to N.
-  if (nb <= 0 || nb > N) nb = N;
+  nb = potrf_block(N, nb);
   // the flags of every panel launch, plus an error word
   const int nleaf = CP_FLAGS * potrf_panel_count(N, nb);
   LeafFlags fl;
   fl.flags = (int*)ws_carve((size_t)(nleaf + 1) * sizeof(int));
-  fl.minv = (double*)ws_carve((size_t)4 * 1024 * sizeof(double));
+  fl.minv = (double*)ws_carve((size_t)CP_MAXB * 1024 * sizeof(double));
   fl.next = 0;
   fl.cap = nleaf;
   if (!fl.flags || !fl.minv) {
