@@ -170,7 +170,7 @@ def test_rankupdatesym(flags, n, k):
 
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
 @pytest.mark.parametrize("n,nb", [(7, 0), (60, 16), (300, 68), (513, 256), (1024, 256), (700, 0),
-                                  (1000, 200), (2500, 256)])
+                                  (1000, 200), (2500, 256), (1536, 384), (1300, 1000), (2100, 0)])
 def test_chol_factor_and_solve(uplo, n, nb):
     A0 = spd(n, 3)
     A = A0.copy(order="F")
@@ -215,3 +215,15 @@ def test_chol_not_spd_panics_with_index():
     A[150, 150] = -1e6
     with pytest.raises(RuntimeError, match=r"illegal value at 150: "):
         mo.DecomposeCHOL(A, mo.LOWER, 64)
+
+
+@pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
+def test_chol_not_spd_in_a_512_column_step(uplo):
+    # blocks of 256 run as 512-column panels: the first bad diagonal (index
+    # 777, in the second panel's fifth 64-column block) is the one reported
+    n = 1200
+    A = spd(n, 3)
+    A[777, 777] = -1e6
+    A = np.asfortranarray(A)
+    with pytest.raises(RuntimeError, match=r"illegal value at 777: "):
+        mo.DecomposeCHOL(A.copy(order="F"), uplo, 256)
