@@ -701,34 +701,46 @@ __global__ __launch_bounds__(256) void k_ldl_panel_rows(int ncol, double* A, lon
   __shared__ double vals[2 * PW * RP_C];
   __shared__ int row[2 * PW];  // row of slot s
   __shared__ int src[2 * PW];  // the row whose value ends in row[s]
+  __shared__ long piv[PW];     // the panel's pivots, loaded by all threads at once
   __shared__ int ns;
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < w; ++s) {
-      row[s] = k0 + s;
-      src[s] = k0 + s;
+  for (int j = threadIdx.x; j < w; j += blockDim.x) piv[j] = ipiv[k0 + j];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // one wave composes the moves in order; a pivot row beyond the panel is
+    // looked up among the slots added so far (at most w of them, one per
+    // lane) with a ballot
+    const int l = threadIdx.x;
+    for (int q = l; q < w; q += 64) {
+      row[q] = k0 + q;
+      src[q] = k0 + q;
     }
     int m = w;
     for (int j = 0; j < w; ++j) {
-      const long pk = ipiv[k0 + j];
+      const long pk = piv[j];
       if (pk <= 0 || pk > n) continue;
       const int pr = (int)pk - 1;
-      int sp = -1;
+      int sp;
       if (pr < k0 + w) {
         sp = pr - k0;
       } else {
-        for (int q = w; q < m; ++q)
-          if (row[q] == pr) sp = q;
-        if (sp < 0) {
+        const unsigned long long hit = __ballot(w + l < m && row[w + l] == pr);
+        if (hit) {
+          sp = w + __ffsll((long long)hit) - 1;
+        } else {
           sp = m++;
-          row[sp] = pr;
-          src[sp] = pr;
+          if (l == 0) {
+            row[sp] = pr;
+            src[sp] = pr;
+          }
         }
       }
-      const int x = src[j];
-      src[j] = src[sp];
-      src[sp] = x;
+      if (l == 0) {
+        const int x = src[j];
+        src[j] = src[sp];
+        src[sp] = x;
+      }
     }
-    ns = m;
+    if (l == 0) ns = m;
   }
   __syncthreads();
   // all sources of this workgroup's RP_C columns are read into LDS before
