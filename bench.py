@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("FP64 DGEMM/DPOTRF TFLOP/s vs peak; mixed-vs-FP64 solve time at 1e-12 bwd err")
 PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 # the trailing-update kernel of the Cholesky (triangle-only GEMM instance)
-SYRK_KERNEL = "k_gemm<double, 64, 64, 16, 2, 2, 1, false, true, true>"
+SYRK_KERNEL = "k_gemm<double, 64, 64, 16, 2, 2, 1, false, true, true, true>"
 LOWER = 0x4
 
 

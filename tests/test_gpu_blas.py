@@ -17,7 +17,8 @@ def _op(X, t):
 
 
 @pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (64, 48, 40), (129, 65, 200), (300, 257, 33)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (64, 48, 40), (129, 65, 200), (300, 257, 33),
+                                   (128, 64, 32), (256, 192, 48)])
 def test_mult_flags(flags, m, n, k):
     ta, tb = bool(flags & mo.TRANSA), bool(flags & mo.TRANSB)
     A = fmat(*((k, m) if ta else (m, k)), seed=11)
@@ -57,7 +58,7 @@ def test_mult_size_mismatch_and_empty():
 
 @pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
 @pytest.mark.parametrize("m,n,k", [(64, 64, 4000), (100, 300, 5000), (1, 1, 2048), (700, 65, 777),
-                                   (1000, 520, 1024)])
+                                   (1000, 520, 1024), (2048, 16, 2048), (2048, 13, 2050)])
 def test_mult_long_k(flags, m, n, k):
     """Few output tiles and a long inner dimension: k is split across
     workgroups and the partial tiles are added in a fixed order (twice the
@@ -92,6 +93,22 @@ def test_rankupdatesym_long_k(flags):
         + 2 * EPS * np.abs(C0)
     assert np.all((np.abs(C - ref) <= bound)[idx])
     np.testing.assert_array_equal(C[other], C0[other])
+
+
+@pytest.mark.parametrize("flags", [mo.NOTRANS, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
+@pytest.mark.parametrize("m,n,k", [(2944, 2944, 48), (2945, 2900, 50)])
+def test_mult_128_tiles(flags, m, n, k):
+    """Outputs of 512 or more 128x128 tiles: the 128x128 instance, with
+    (2944 = 23 x 128, k = 48) and without bounds checks compiled in."""
+    ta, tb = bool(flags & mo.TRANSA), bool(flags & mo.TRANSB)
+    A = fmat(*((k, m) if ta else (m, k)), seed=41)
+    B = fmat(*((n, k) if tb else (k, n)), seed=42)
+    C0 = fmat(m, n, seed=43)
+    C = C0.copy(order="F")
+    assert mo.Mult(C, A, B, 1.5, -0.5, flags) is None
+    ref = -0.5 * C0 + 1.5 * (_op(A, ta) @ _op(B, tb))
+    bound = 1.5 * gemm_bound(np.abs(_op(A, ta)), np.abs(_op(B, tb)), k) + 2 * EPS * np.abs(0.5 * C0)
+    assert np.all(np.abs(C - ref) <= bound)
 
 
 def test_mult_dgemm_1024():
@@ -149,7 +166,7 @@ def test_solvetrm_all_variants(flags, n, nrhs):
 
 
 @pytest.mark.parametrize("flags", [mo.LOWER, mo.UPPER, mo.LOWER | mo.TRANSA, mo.UPPER | mo.TRANSA])
-@pytest.mark.parametrize("n,k", [(1, 1), (100, 70), (515, 256)])
+@pytest.mark.parametrize("n,k", [(1, 1), (100, 70), (515, 256), (512, 256), (192, 64)])
 def test_rankupdatesym(flags, n, k):
     ta = bool(flags & mo.TRANSA)
     A = fmat(*((k, n) if ta else (n, k)), seed=8)

@@ -45,7 +45,7 @@ def _oracle(fn, *args):
 
 @pytest.mark.parametrize("side", [mo.LEFT, mo.RIGHT])
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
-@pytest.mark.parametrize("n,m", [(1, 1), (90, 70), (136, 136), (700, 333)])
+@pytest.mark.parametrize("n,m", [(1, 1), (90, 70), (136, 136), (700, 333), (256, 128)])
 def test_multsym(side, uplo, n, m):
     if side == mo.RIGHT:
         m = n  # the reference's size check admits RIGHT for square B only
@@ -100,7 +100,7 @@ def _tri_operand(n, upper, unit, seed):
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
 @pytest.mark.parametrize("trans", [0, mo.TRANSA])
 @pytest.mark.parametrize("unit", [0, mo.UNIT])
-@pytest.mark.parametrize("n,m", [(80, 33), (700, 150)])
+@pytest.mark.parametrize("n,m", [(80, 33), (700, 150), (256, 192)])
 def test_multtrm(side, uplo, trans, unit, n, m):
     A, T = _tri_operand(n, uplo == mo.UPPER, unit, 31)
     opT = T.T if trans else T
@@ -121,7 +121,7 @@ def test_multtrm(side, uplo, trans, unit, n, m):
 
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
 @pytest.mark.parametrize("trans", [0, mo.TRANSA])
-@pytest.mark.parametrize("n,k", [(150, 40), (1000, 256)])
+@pytest.mark.parametrize("n,k", [(150, 40), (1000, 256), (512, 128)])
 def test_rankupdate2sym(uplo, trans, n, k):
     A = fmat(*((k, n) if trans else (n, k)), seed=41)
     B = fmat(*((k, n) if trans else (n, k)), seed=42)
@@ -145,7 +145,7 @@ def test_rankupdate2sym(uplo, trans, n, k):
 
 @pytest.mark.parametrize("uplo", [mo.LOWER, mo.UPPER])
 @pytest.mark.parametrize("flags", [0, mo.TRANSA, mo.TRANSB, mo.TRANSA | mo.TRANSB])
-@pytest.mark.parametrize("n,k", [(140, 50), (900, 300)])
+@pytest.mark.parametrize("n,k", [(140, 50), (900, 300), (384, 64)])
 def test_updatetrm(uplo, flags, n, k):
     ta, tb = flags & mo.TRANSA, flags & mo.TRANSB
     A = fmat(*((k, n) if ta else (n, k)), seed=61)
