@@ -248,29 +248,27 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
       }
     }
     if (diag) {
+      // own diagonal block: dacc -= X X^T (with lanes along the rows the
+      // result lands in the layout of dacc: the block is symmetric); its
+      // MFMAs run while the stores of X drain, ahead of the flag
+#pragma unroll
+      for (int s4 = 0; s4 < LEAF; s4 += 4) {
+        const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+          dacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[(16 * tj + li) * CL_LD + s4 + lq], xa,
+                                                          dacc[tj], 0, 0, 0);
+      }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (t == 0) __hip_atomic_store(&xflag[CP_MAXB * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // updates of the column blocks right of j, right to left, so that the
-    // block of column j + 1 (the next right-hand side) comes last
-    const int chi = diag ? b : nsteps - 1;
+    // updates of the other column blocks right of j, right to left, so that
+    // the block of column j + 1 (the next right-hand side) comes last
+    const int chi = diag ? b - 1 : nsteps - 1;
 #pragma unroll 1
     for (int cb = chi; cb > j; --cb) {
       const int cc0 = LEAF * cb, wcb = min(LEAF, pw - cc0);
-      if (diag && cb == b) {
-        // own diagonal block: dacc -= X X^T; with lanes along the rows the
-        // result lands in the layout of dacc (the block is symmetric)
-#pragma unroll
-        for (int s4 = 0; s4 < LEAF; s4 += 4) {
-          const double xa = -Bs[(R0 + li) * CL_LD + s4 + lq];
-#pragma unroll
-          for (int tj = 0; tj < 4; ++tj)
-            dacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[(16 * tj + li) * CL_LD + s4 + lq], xa,
-                                                            dacc[tj], 0, 0, 0);
-        }
-        continue;
-      }
       if (t == 0) cp_wait(&xflag[CP_MAXB * j + cb], err);
       __syncthreads();
       {
@@ -375,16 +373,21 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
           ri = 1.0 / r;
         }
         const double x = l > j ? a[ji] * ri : 0.0;
+        double xn = 0.0;  // x of row j + 1 (= Lt[j * 64 + j + 1])
         if (ji < 15) {
-          const double xn = readlane_d(x, j + 1);
+          xn = readlane_d(x, j + 1);
           dn = fma(-xn, xn, readlane_d(a[ji + 1], j + 1));
         }
         a[ji] = l == j ? r : x;
         Lt[j * 64 + l] = a[ji];
         dv[j] = d;
         rv[j] = ri;
+        // the next column's update takes x of row j + 1 from the readlane
+        // rather than back from LDS (the read after the store above sat on
+        // the serial path); the later columns read it from LDS
 #pragma unroll
-        for (int i = ji + 1; i < 16; ++i) a[i] = fma(-x, Lt[j * 64 + 16 * jb + i], a[i]);
+        for (int i = ji + 1; i < 16; ++i)
+          a[i] = fma(-x, i == ji + 1 ? xn : Lt[j * 64 + 16 * jb + i], a[i]);
         // keep this column's updates here: sinking them toward their use
         // would put chains of dependent FMAs on the serial path
         __builtin_amdgcn_sched_barrier(0);
@@ -610,15 +613,17 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
     // step k to the rest of the trailing matrix. The rest update starts only
     // when the block column update is done, at the moment the next panel
     // becomes ready too: both kernels then enter the dispatcher together,
-    // and the panel's workgroups are not left waiting for CU slots behind
-    // thousands of queued tiles. The rest update does block column k+2
-    // first (a trapezoid, as soon as panel k is done) and then the triangle
+    // the panel's workgroups are not left waiting for CU slots behind
+    // thousands of queued tiles, and the block column update, on the panel
+    // chain, has the GPU to itself (Cholesky n = 8192: 29.45 -> 29.7
+    // TFLOP/s against starting block column k+2 beside it). The rest update
+    // does block column k+2 first (a trapezoid) and then the triangle
     // beyond it: the next block column update needs only the first part, so
     // the panel chain does not wait for the whole triangle.
     hipStream_t s2 = aux_stream();
     hipEvent_t e_fork = sync_event(0), e_col = sync_event(1), e_next = sync_event(2),
-               e_join = sync_event(3), e_pan = sync_event(4);
-    if (!s2 || !e_fork || !e_col || !e_next || !e_join || !e_pan) {
+               e_join = sync_event(3);
+    if (!s2 || !e_fork || !e_col || !e_next || !e_join) {
       set_error("potrf: could not create the look-ahead stream");
       return ST_HIP;
     }
@@ -631,7 +636,6 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       const int m = N - k - kb;
       if (m <= 0) break;
       const int kb1 = min(nb, m);  // width of the next block column
-      MOPS_CHECK_HIP(hipEventRecord(e_pan, s2));  // panel k done
       // s2 may touch block column k+1 only after step k-1 has updated it
       if (have_next) MOPS_CHECK_HIP(hipStreamWaitEvent(s2, e_next, 0));
       MOPS_TRY(chol_update(s2, upper, N, k, kb, 0, kb1, true, A, lda));
@@ -640,13 +644,12 @@ int potrf_d(hipStream_t s, int flags, int N, int nb, double* A, long lda, int* i
       if (m > kb1) {
         const int mr = m - kb1;
         const int kb2 = min(nb, mr);  // block column k+2
-        // block column k+2 needs panel k only; the triangle waits for the
-        // block column update, to start together with the next panel
-        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_pan, 0));
+        // block column k+2 (which needs panel k only) and the triangle wait
+        // for the block column update, to start together with the next panel
+        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
         MOPS_TRY(chol_update(s, upper, N, k, kb, kb1, kb2, true, A, lda));
         MOPS_CHECK_HIP(hipEventRecord(e_next, s));
         have_next = true;
-        MOPS_CHECK_HIP(hipStreamWaitEvent(s, e_col, 0));
         if (mr > kb2) {
           // timed when tracing: every launch of the triangle-only GEMM
           const int mq = mr - kb2;
