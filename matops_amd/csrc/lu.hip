@@ -19,12 +19,14 @@
 // Panel (columns k..k+kb, rows k..m-1), right-looking in 32-column leaves.
 // A leaf is one kernel of G co-resident workgroups of 512 threads, one row
 // per thread held in registers. Per column every workgroup finds its best
-// candidate (wave shuffles) and publishes it with the candidate row into
-// exchange records that start out as a sentinel; every workgroup polls the
-// candidates' keys until all are there, agrees on the pivot, loads the
-// pivot row and the old row k, exchanges the two rows in registers and
-// applies the rank-1 update to its rows (no grid barrier per column). The
-// interchanges of
+// candidate (wave shuffles) and publishes its key into exchange records
+// that start out as a sentinel, then the candidate row; every workgroup
+// polls the candidates' keys until all are there, agrees on the pivot,
+// loads the pivot row and the old row k, exchanges the two rows in
+// registers and updates the next column (no grid barrier per column); the
+// rest of the rank-1 update runs in the next column while its keys travel
+// (n = 16384: 118.7 -> 116.8 ms against the whole update before the
+// publish). The interchanges of
 // the leaf's rows in the other panel columns are applied once, at the end
 // of the leaf, as a gather of the rows that moved. After each leaf the
 // U block right of it is solved with the leaf's unit triangle (one small
@@ -58,7 +60,7 @@ inline int lu_panel_width(int nb, int mn, int wmax) {
   const int w = (nb <= 0 || nb >= 256) ? wmax : nb;
   return w < mn ? w : mn;
 }
-constexpr int LEAF_SW = LEAF_W + 3;  // exchange slot: value, row, row data, perm
+constexpr int LEAF_SW = LEAF_W + 4;  // exchange slot: key, row, row data, perm, state
 constexpr unsigned SPIN_LIMIT = 1u << 26;
 // Panel state (row movement, inverse of the unit triangle) rotates over
 // three slots: the rest update of step k may still read slot k mod 3 while
@@ -103,8 +105,8 @@ __device__ __forceinline__ bool is_xsent(double v) {
   return (unsigned long long)__double_as_longlong(v) == XSENT;
 }
 // One column's exchange records: LEAF_GMAX candidate slots [key, row, row
-// data, perm], then the old row k [data, perm, key]. Columns use a ring of
-// LEAF_RING such areas.
+// data, perm, state], then the old row k [data, perm, state]. Columns use a
+// ring of LEAF_RING such areas.
 constexpr int LEAF_XAREA = LEAF_GMAX * LEAF_SW + LEAF_W + 2;
 constexpr int LEAF_RING = 3;
 
@@ -148,6 +150,14 @@ __device__ __forceinline__ void grid_sync(unsigned* ctr, unsigned target, int* e
     }
   }
   __syncthreads();
+}
+
+// True, every 1024 polls, once some wait of the factorization has timed out
+// (the error word is set): later waits then give up at once instead of
+// each running to the spin limit.
+__device__ __forceinline__ bool leaf_failed(const int* err, unsigned spins) {
+  return (spins & 1023u) == 0 &&
+         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 // One DPP lane move of a double (both halves); lanes whose source is out of
@@ -197,7 +207,7 @@ __device__ __forceinline__ int wave_first_max(double v, double& vmax) {
 // the panel move later, all at once, from the permutation).
 __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
   constexpr int W = LEAF_W, SW = LEAF_SW, NWV = LEAF_R / 64;
-  __shared__ double xs[2 * (W + 1)];  // the pivot row, then the old row k
+  __shared__ double xs[2][2 * (W + 1)];  // by column parity: the pivot row, then the old row k
   __shared__ int pinfo;               // the pivot of the current column
   __shared__ double wv[NWV];
   __shared__ int wr[NWV];
@@ -222,6 +232,14 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
   if (mine) pm = a.perm[row];
   unsigned target = a.base;
 
+  // The rank-1 update of column j reaches only column j + 1 in step 7; the
+  // columns after it get it in step 3 of column j + 1, after the key of
+  // column j + 1 is out, so it runs while the keys travel. dpend: this row
+  // takes that pending update, with multiplier dl and the pivot row of the
+  // previous column (xs[(j - 1) & 1]).
+  bool dpend = false;
+  double dl = 0.0;
+
 #pragma unroll 1
   for (int j = 0; j < w; ++j) {
     const int k = c0 + j;
@@ -231,9 +249,9 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
     // once every workgroup is done reading it.
     double* xaj = a.xa + (long)((a.ring0 + j) % LEAF_RING) * LEAF_XAREA;
     double* xareset = a.xa + (long)((a.ring0 + j + 2) % LEAF_RING) * LEAF_XAREA;
-    // 1. candidate of each wave; the winning lane parks its row in LDS.
-    // Keys: |a|; a NaN is -1 (never chosen) except on the diagonal, where
-    // it is +inf (nothing replaces it); rows that are not candidates -2.
+    // 1. candidate of each wave (x[0], column j, is final). Keys: |a|; a
+    // NaN is -1 (never chosen) except on the diagonal, where it is +inf
+    // (nothing replaces it); rows that are not candidates -2.
     double v = -2.0;
     if (mine && row >= k) {
       const double ax = fabs(x[0]);
@@ -241,14 +259,43 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
     }
     double vmax;
     const int wl = wave_first_max(v, vmax);
+    if (lane == 0) {
+      wv[wave] = vmax;
+      wr[wave] = vmax > -2.0 ? rg0 + 64 * wave + wl : INT_MAX;
+    }
+    __syncthreads();
+    // 2. wave 0 publishes this workgroup's key and candidate row index (the
+    // row data follows in step 5). The wait first orders every earlier
+    // store of this wave (the previous column's data, the sentinels put
+    // back into a ring area) before anything of this column.
+    int bq = 0;  // the workgroup's winning wave (wave 0 only)
+    if (wave == 0) {
+      __builtin_amdgcn_s_waitcnt(0);
+      double bv;
+      bq = wave_first_max(lane < NWV ? wv[lane] : -3.0, bv);
+      const int br = wr[bq];
+      double* slot = xaj + (long)g * SW;
+      if (lane == 0) {
+        st_agent(&slot[1], (double)br);
+        st_agent(&slot[0], bv);
+      }
+    }
+    // 3. the pending update of column j - 1 on the columns after j
+    if (dpend) {
+      const double* pp = xs[(j - 1) & 1];  // pp[c + 1] belongs to x[c]
+      const int nd = __builtin_amdgcn_readfirstlane(w - 1 - j);
+#pragma unroll
+      for (int c = 1; c < W; ++c) {
+        // computed for every c, kept only for the pending columns (no branch)
+        const double nv = fma(-dl, pp[c + 1], x[c]);
+        x[c] = c <= nd ? nv : x[c];
+      }
+    }
+    // 4. the final rows: each wave's candidate row and the old row k
     if (lane == wl) {
 #pragma unroll
       for (int c = 0; c < W; ++c) wrow[wave][c] = x[c];
       wrow[wave][W] = (double)pm;
-    }
-    if (lane == 0) {
-      wv[wave] = vmax;
-      wr[wave] = vmax > -2.0 ? rg0 + 64 * wave + wl : INT_MAX;
     }
     if (row == k) {
 #pragma unroll
@@ -256,27 +303,18 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
       okl[W] = (double)pm;
     }
     __syncthreads();
-    // 2. publish this workgroup's candidate (and the old row k). The key
-    // and the row index are never the sentinel; row data that happens to
-    // equal it is published "escaped": data first, then (after the stores
-    // completed) the row index stored as -row-1, so readers know the data
-    // was complete before the key. The wait at the start orders the
-    // sentinel this wave put back into a ring area before anything of this
-    // column (a reader of this column's records then finds either the
-    // sentinel or the new record there).
+    // 5. the row data: the candidate row (wave 0) and the old row k (wave 1
+    // of the workgroup that holds it). Data that happens to equal the
+    // sentinel is published "escaped": a state word, stored after the data
+    // stores completed, tells readers the data is complete.
     if (wave == 0) {
-      __builtin_amdgcn_s_waitcnt(0);
-      double bv;
-      const int bq = wave_first_max(lane < NWV ? wv[lane] : -3.0, bv);
-      const int br = wr[bq];
       double* slot = xaj + (long)g * SW;
       const double dv = lane <= W ? wrow[bq][lane] : 0.0;
       const bool esc = __ballot(lane <= W && is_xsent(dv)) != 0;
       if (lane <= W) st_agent(&slot[2 + lane], dv);
-      if (esc) __builtin_amdgcn_s_waitcnt(0);
-      if (lane == 0) {
-        st_agent(&slot[1], esc ? -(double)br - 1.0 : (double)br);
-        st_agent(&slot[0], bv);
+      if (esc) {
+        __builtin_amdgcn_s_waitcnt(0);
+        if (lane == 0) st_agent(&slot[SW - 1], 2.0);
       }
     }
     if (wave == 1 && k >= rg0 && k < rg0 + LEAF_R) {
@@ -287,107 +325,106 @@ __global__ __launch_bounds__(LEAF_R) void k_getf2_leaf(LeafArgs a) {
       if (esc) __builtin_amdgcn_s_waitcnt(0);
       if (lane == 0) st_agent(&rk[W + 1], esc ? 2.0 : 1.0);
     }
-    // 3. wave 0 waits for every slot's key and row index and for the old
-    // row k's key (one slot per lane: a few words per workgroup are polled),
-    // chooses the pivot (the first workgroup holding the largest key;
-    // workgroups are in row order), then loads the pivot row and the old
-    // row k, again while they hold the sentinel unless escaped. One store
-    // and two load round trips per column, no grid barrier.
+    // 6. wave 0 waits for every slot's key and row index (one slot per
+    // lane), chooses the pivot (the first workgroup holding the largest key;
+    // workgroups are in row order), then loads the pivot row and the old row
+    // k until they hold no sentinel, or until their state word says they
+    // were published escaped (then they are loaded once more). One store and
+    // two load round trips per column, no grid barrier.
+    double* xsj = xs[j & 1];
     if (wave == 0) {
       const bool has = lane < G;
-      double kv = -3.0, rv = 0.0, rkk = 1.0;
+      double kv = -3.0, rv = 0.0;
       if (has) {
         kv = ld_agent(&xaj[(long)lane * SW]);
         rv = ld_agent(&xaj[(long)lane * SW + 1]);
       }
-      if (lane == 0) rkk = ld_agent(&xaj[(long)LEAF_GMAX * SW + W + 1]);
       unsigned spins = 0;
       for (;;) {
-        const bool pend = (has && (is_xsent(kv) || is_xsent(rv))) || (lane == 0 && is_xsent(rkk));
+        const bool pend = has && (is_xsent(kv) || is_xsent(rv));
         if (__ballot(pend) == 0) break;
-        if (++spins > SPIN_LIMIT) {
+        if (++spins > SPIN_LIMIT || leaf_failed(a.err, spins)) {
           if (lane == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
         if (has && is_xsent(kv)) kv = ld_agent(&xaj[(long)lane * SW]);
         if (has && is_xsent(rv)) rv = ld_agent(&xaj[(long)lane * SW + 1]);
-        if (lane == 0 && is_xsent(rkk)) rkk = ld_agent(&xaj[(long)LEAF_GMAX * SW + W + 1]);
       }
       double bv;
-      const int bq = wave_first_max(kv, bv);
-      const double brw = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rv), bq),
-                                          __builtin_amdgcn_readlane(__double2loint(rv), bq));
-      const double rk_key = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rkk), 0),
-                                             __builtin_amdgcn_readlane(__double2loint(rkk), 0));
-      const bool cand = bv > -2.0;            // some row >= k exists (always, as k < m)
-      const bool pwait = cand && brw >= 0.0;  // pivot row not escaped: poll its data
-      const bool owait = rk_key == 1.0;       // old row k not escaped
-      double pd = 0.0, od = 0.0;
+      const int pq = wave_first_max(kv, bv);
+      const double brw = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rv), pq),
+                                          __builtin_amdgcn_readlane(__double2loint(rv), pq));
+      const bool cand = bv > -2.0;  // some row >= k exists (always, as k < m)
+      const double* pslot = xaj + (long)pq * SW;
+      const double* rk = xaj + (long)LEAF_GMAX * SW;
+      double pd = 0.0, od = 0.0, pst = 0.0, ost = 0.0;
       if (lane <= W) {
-        if (cand) pd = ld_agent(&xaj[(long)bq * SW + 2 + lane]);
-        od = ld_agent(&xaj[(long)LEAF_GMAX * SW + lane]);
+        if (cand) pd = ld_agent(&pslot[2 + lane]);
+        od = ld_agent(&rk[lane]);
       }
+      if (lane == 0) {
+        if (cand) pst = ld_agent(&pslot[SW - 1]);
+        ost = ld_agent(&rk[W + 1]);
+      }
+      bool pesc = false, oesc = false;  // escaped and complete
       for (;;) {
-        const bool pend = lane <= W && ((pwait && is_xsent(pd)) || (owait && is_xsent(od)));
+        pesc = __builtin_amdgcn_readfirstlane(__double2hiint(pst)) == __double2hiint(2.0) &&
+               __builtin_amdgcn_readfirstlane(__double2loint(pst)) == 0;
+        oesc = __builtin_amdgcn_readfirstlane(__double2hiint(ost)) == __double2hiint(2.0) &&
+               __builtin_amdgcn_readfirstlane(__double2loint(ost)) == 0;
+        const bool pend =
+            lane <= W && ((cand && !pesc && is_xsent(pd)) || (!oesc && is_xsent(od)));
         if (__ballot(pend) == 0) break;
-        if (++spins > SPIN_LIMIT) {
+        if (++spins > SPIN_LIMIT || leaf_failed(a.err, spins)) {
           if (lane == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        if (lane <= W && pwait && is_xsent(pd)) pd = ld_agent(&xaj[(long)bq * SW + 2 + lane]);
-        if (lane <= W && owait && is_xsent(od)) od = ld_agent(&xaj[(long)LEAF_GMAX * SW + lane]);
+        if (lane <= W && cand && !pesc && is_xsent(pd)) pd = ld_agent(&pslot[2 + lane]);
+        if (lane <= W && !oesc && is_xsent(od)) od = ld_agent(&rk[lane]);
+        if (lane == 0 && cand && !pesc) pst = ld_agent(&pslot[SW - 1]);
+        if (lane == 0 && !oesc) ost = ld_agent(&rk[W + 1]);
       }
+      // escaped data may have been loaded before its last stores landed:
+      // load it again, now that the state word has been seen
+      if (lane <= W && cand && pesc) pd = ld_agent(&pslot[2 + lane]);
+      if (lane <= W && oesc) od = ld_agent(&rk[lane]);
       if (lane <= W) {
-        xs[lane] = pd;
-        xs[W + 1 + lane] = od;
+        xsj[lane] = pd;
+        xsj[W + 1 + lane] = od;
       }
-      if (lane == 0) {
-        const int br = (int)(brw < 0.0 ? -brw - 1.0 : brw);
-        pinfo = cand ? br : k;
-      }
-      // every workgroup has published this column, so all of them are done
-      // reading the previous one: its area gets the sentinel back
+      if (lane == 0) pinfo = cand ? (int)brw : k;
+      // every workgroup has published this column's key, so all of them are
+      // done reading the previous column: its area gets the sentinel back
       const double sv = __longlong_as_double((long long)XSENT);
       if (lane < SW) st_agent(&xareset[(long)g * SW + lane], sv);
       if (g == 0 && lane < W + 2) st_agent(&xareset[(long)LEAF_GMAX * SW + lane], sv);
     }
     __syncthreads();
-    // 4. the pivot, the pivot row and the old row k, for every thread
+    // 7. the pivot, the interchange in registers, the column of L and the
+    // update of column j + 1 (the rest waits for step 3 of the next column)
     const int p = pinfo;
     if (t == 0) {
       pv[j] = p;
       if (g == 0) a.piv[k] = p;
     }
-    const double* prow = xs;          // the pivot row; prow[W] = its perm
-    const double* orow = xs + W + 1;  // the old row k
-    // 5. interchange in registers
+    const double* prow = xsj;          // the pivot row; prow[W] = its perm
+    const double* orow = xsj + W + 1;  // the old row k
     if (p != k && (row == p || row == k)) {
       const double* src = row == p ? orow : prow;
 #pragma unroll
       for (int c = 0; c < W; ++c) x[c] = src[c];
       pm = (int)src[W];
     }
-    // 6. column of L and rank-1 update of the rows below k (columns j+1 ..
-    // w-1 are x[1 .. w-1-j]; the rest hold finished columns or padding)
-    if (mine && row > k) {
-      // the whole pivot row into registers first (every LDS read in flight
-      // together), then the divide and the updates
-      double p[W];
-#pragma unroll
-      for (int c = 0; c < W; ++c) p[c] = prow[c];
-      const double l = x[0] * (1.0 / p[0]);
+    dpend = mine && row > k;
+    if (dpend) {
+      const double l = x[0] * (1.0 / prow[0]);
       x[0] = l;
-      const int nc = __builtin_amdgcn_readfirstlane(w - 1 - j);
-#pragma unroll
-      for (int c = 1; c < W; ++c) {
-        // computed for every c, kept only for the active columns (no branch)
-        const double nv = fma(-l, p[c], x[c]);
-        x[c] = c <= nc ? nv : x[c];
-      }
+      if (j + 1 < w) x[1] = fma(-l, prow[1], x[1]);
+      dl = l;
     }
-    // 7. rotate: column j moves to the back
+    // rotate: column j moves to the back
     const double x0 = x[0];
 #pragma unroll
     for (int c = 0; c < W - 1; ++c) x[c] = x[c + 1];
