@@ -70,6 +70,37 @@ __global__ __launch_bounds__(256) void k_col_absmax(int n, const double* __restr
   if (threadIdx.x == 0) out[j] = red[0];
 }
 
+// bits[z * nrhs + j] = max over the rows of |M[i][j]| for M = R (z = 0)
+// and X (z = 1), as the bits of a non-negative double (a NaN is stored as
+// the quiet NaN, whose bits exceed those of +Inf, so the maximum keeps it):
+// 2048 rows per workgroup, the bits cleared beforehand
+constexpr int ABSMAX_ROWS = 2048;
+__global__ __launch_bounds__(256) void k_absmax_rx(int n, int nrhs, const double* __restrict__ R,
+                                                   long ldr, const double* __restrict__ X, long ldx,
+                                                   unsigned long long* bits) {
+  __shared__ double red[256];
+  const int j = blockIdx.y;
+  const double* M = blockIdx.z ? X + (long)j * ldx : R + (long)j * ldr;
+  const int i0 = blockIdx.x * ABSMAX_ROWS, i1 = min(n, i0 + ABSMAX_ROWS);
+  double m = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) m = max_nan(m, fabs(M[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = max_nan(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v = red[0];
+    atomic_max_nonneg(&bits[blockIdx.z * nrhs + j],
+                      v != v ? __longlong_as_double(0x7ff8000000000000ll) : v);
+  }
+}
+
+__global__ void k_zero_u64n(unsigned long long* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
 __global__ void k_add_tile(int n, int nrhs, const double* __restrict__ D, long ldd, double* X,
                            long ldx) {
   const long total = (long)n * nrhs;
@@ -91,19 +122,24 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 // eta = max_j rmax_j / (anorm xmax_j + bmax_j); +Inf when any of the norms
 // is not finite (a singular A gives Inf/NaN in X, which must never read as
-// converged)
-__global__ void k_eta(int nrhs, const double* rmax, const double* xmax, const double* bmax,
+// converged). rmax and xmax come as k_absmax_rx's bits, which are cleared
+// here for the next iteration.
+__global__ void k_eta(int nrhs, unsigned long long* bits, const double* bmax,
                       const unsigned long long* anorm_bits, double* eta) {
   if (threadIdx.x != 0) return;
   const double an = __longlong_as_double((long long)*anorm_bits);
   double e = isfinite(an) ? 0.0 : INFINITY;
   for (int j = 0; j < nrhs; ++j) {
-    if (!isfinite(rmax[j]) || !isfinite(xmax[j]) || !isfinite(bmax[j])) {
+    const double rmax = __longlong_as_double((long long)bits[j]);
+    const double xmax = __longlong_as_double((long long)bits[nrhs + j]);
+    bits[j] = 0ull;
+    bits[nrhs + j] = 0ull;
+    if (!isfinite(rmax) || !isfinite(xmax) || !isfinite(bmax[j])) {
       e = INFINITY;
       continue;
     }
-    const double den = an * xmax[j] + bmax[j];
-    const double q = den > 0.0 ? rmax[j] / den : (rmax[j] > 0.0 ? INFINITY : 0.0);
+    const double den = an * xmax + bmax[j];
+    const double q = den > 0.0 ? rmax / den : (rmax > 0.0 ? INFINITY : 0.0);
     e = max_nan(e, q);
   }
   *eta = e;
@@ -120,10 +156,10 @@ size_t gesv_ir_ws_bytes(int n, int nrhs, int prec) {
   if (n <= 0 || nrhs <= 0) return 0;
   size_t b = (size_t)n * n * sizeof(double);          // LU
   b += 2 * (size_t)n * nrhs * sizeof(double);         // R, D
-  b += (size_t)n * sizeof(int) + 4 * nrhs * sizeof(double) + 4096;
+  b += (size_t)n * sizeof(int) + 6 * nrhs * sizeof(double) + 4096;
   b += (size_t)NORM_CHUNKS * n * sizeof(double);     // row-sum chunks of the norm
   const size_t f = prec != 0 ? getrf_mixed_ws_bytes(n, 256) : getrf_ws_bytes(n, n, 256);
-  const size_t r = getrs_ws_bytes(n, nrhs) + gemm_split_ws_bytes(n, nrhs, n, sizeof(double));
+  const size_t r = getrs_prep_ws_bytes(n, nrhs) + gemm_split_ws_bytes(n, nrhs, n, sizeof(double));
   return b + (f > r ? f : r);
 }
 
@@ -145,14 +181,15 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
   double* D = (double*)ws_carve((size_t)n * nrhs * sizeof(double));
   int* perm = (int*)ws_carve((size_t)n * sizeof(int));
   double* cm = (double*)ws_carve(4 * (size_t)nrhs * sizeof(double) + 64);
+  unsigned long long* bits = (unsigned long long*)ws_carve(2 * (size_t)nrhs * sizeof(double) + 64);
   unsigned long long* anorm = (unsigned long long*)ws_carve(64);
   int* err = (int*)ws_carve(64);
   double* part = (double*)ws_carve((size_t)NORM_CHUNKS * n * sizeof(double));
-  if (!LU || !R || !D || !perm || !cm || !anorm || !err || !part) {
+  if (!LU || !R || !D || !perm || !cm || !bits || !anorm || !err || !part) {
     set_error("gesv_ir: workspace too small");
     return ST_NOMEM;
   }
-  double *rmax = cm, *xmax = cm + nrhs, *bmax = cm + 2 * nrhs, *eta = cm + 3 * nrhs;
+  double *bmax = cm + 2 * nrhs, *eta = cm + 3 * nrhs;
   // own events: the factorizations use the shared synchronization events
   struct Events {
     hipEvent_t e[3] = {nullptr, nullptr, nullptr};
@@ -186,22 +223,21 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
   hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, B, ldb, bmax);
   hipLaunchKernelGGL(k_zero_tile, dim3(grid_for((long)n * nrhs)), dim3(256), 0, s, n, nrhs, X, ldx);
   MOPS_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_u64n, dim3(1), dim3(256), 0, s, bits, 2 * nrhs);
   MOPS_TRY(copy_tile_d(s, n, nrhs, B, ldb, R, n));
+  // the inverses of the diagonal blocks of L and U, once for every solve
+  const size_t pmark = ws_mark();
+  GetrsPrep gp;
+  MOPS_TRY(getrs_prep_d(s, n, nrhs, LU, n, gp));
+  const dim3 amax_grid(ceil_div(n, ABSMAX_ROWS), nrhs, 2);
   for (int it = 1; it <= maxit; ++it) {
-    MOPS_TRY(copy_tile_d(s, n, nrhs, R, n, D, n));
-    {
-      const size_t mark = ws_mark();
-      const int st = getrs_d(s, 0, n, nrhs, LU, n, nullptr, 0, perm, D, n);
-      ws_release(mark);
-      MOPS_TRY(st);
-    }
+    MOPS_TRY(getrs_prep_solve_d(s, n, nrhs, LU, n, perm, R, n, D, n, gp));
     hipLaunchKernelGGL(k_add_tile, dim3(grid_for((long)n * nrhs)), dim3(256), 0, s, n, nrhs, D,
                        (long)n, X, ldx);
     MOPS_TRY(copy_tile_d(s, n, nrhs, B, ldb, R, n));
     MOPS_TRY(gemm_d(s, false, false, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, n));
-    hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, R, (long)n, rmax);
-    hipLaunchKernelGGL(k_col_absmax, dim3(nrhs), dim3(256), 0, s, n, X, ldx, xmax);
-    hipLaunchKernelGGL(k_eta, dim3(1), dim3(1), 0, s, nrhs, rmax, xmax, bmax, anorm, eta);
+    hipLaunchKernelGGL(k_absmax_rx, amax_grid, dim3(256), 0, s, n, nrhs, R, (long)n, X, ldx, bits);
+    hipLaunchKernelGGL(k_eta, dim3(1), dim3(1), 0, s, nrhs, bits, bmax, anorm, eta);
     MOPS_CHECK_HIP(hipGetLastError());
     double e = 0.0;
     MOPS_CHECK_HIP(hipMemcpyAsync(&e, eta, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -214,6 +250,7 @@ int gesv_ir_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const d
       break;
     }
   }
+  ws_release(pmark);
   MOPS_CHECK_HIP(hipEventRecord(e2, s));
   MOPS_CHECK_HIP(hipEventSynchronize(e2));
   float f_ms = 0.f, r_ms = 0.f;

@@ -1159,6 +1159,30 @@ int getrf_mixed_d(hipStream_t s, int n, int nb, const double* A, long lda, doubl
   return ST_OK;
 }
 
+size_t getrs_prep_ws_bytes(int n, int nrhs) {
+  // two preparations, and trsm_d's own workspace when the solve is not chained
+  return 2 * trsm_prep_ws_bytes(n, nrhs) + trsm_ws_bytes(F_LEFT, n, nrhs) + 512;
+}
+
+int getrs_prep_d(hipStream_t s, int n, int nrhs, const double* A, long lda, GetrsPrep& g) {
+  g = GetrsPrep();
+  if (n <= 0 || nrhs <= 0) return ST_OK;
+  MOPS_TRY(trsm_prep_d(s, F_LEFT | F_LOWER | F_UNIT, n, nrhs, A, lda, nullptr, g.l));
+  // one sentinel copy serves both solves (they run one after the other)
+  return trsm_prep_d(s, F_LEFT | F_UPPER, n, nrhs, A, lda, g.l.W, g.u);
+}
+
+int getrs_prep_solve_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const int* perm,
+                       const double* R, long ldr, double* Out, long ldo, const GetrsPrep& g) {
+  if (n <= 0 || nrhs <= 0) return ST_OK;
+  // SolveLU (lu.go:320-330): pivots, then L (unit lower), then U
+  hipLaunchKernelGGL(k_gather_rows, dim3(grid1((long)n * nrhs)), dim3(256), 0, s, n, nrhs, perm,
+                     R, ldr, Out, ldo);
+  MOPS_CHECK_HIP(hipGetLastError());
+  MOPS_TRY(trsm_prep_solve_d(s, F_LEFT | F_LOWER | F_UNIT, n, nrhs, A, lda, Out, ldo, g.l));
+  return trsm_prep_solve_d(s, F_LEFT | F_UPPER, n, nrhs, A, lda, Out, ldo, g.u);
+}
+
 size_t getrs_ws_bytes(int n, int nrhs) {
   return (size_t)n * nrhs * sizeof(double) + (size_t)n * sizeof(int) + 512 +
          2 * trsm_ws_bytes(F_LEFT, n, nrhs);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "mops_kernels.hpp"
+#include "mops_solvers.hpp"
 
 namespace mops {
 
@@ -18,6 +19,19 @@ size_t getrs_ws_bytes(int n, int nrhs);
 // perm may be null: it is then rebuilt from the npiv pivots.
 int getrs_d(hipStream_t s, int flags, int n, int nrhs, const double* A, long lda,
             const long long* ipiv, int npiv, const int* perm, double* B, long ldb);
+
+// getrs_d without TRANSA, repeated with one set of factors (iterative
+// refinement): getrs_prep_d computes the inverses of the diagonal blocks of
+// L and U once (workspace getrs_prep_ws_bytes, held until the last solve);
+// getrs_prep_solve_d then sets Out = A^-1 R (R is read only; Out may not
+// alias R) with them.
+struct GetrsPrep {
+  TrsmPrep l, u;
+};
+size_t getrs_prep_ws_bytes(int n, int nrhs);
+int getrs_prep_d(hipStream_t s, int n, int nrhs, const double* A, long lda, GetrsPrep& g);
+int getrs_prep_solve_d(hipStream_t s, int n, int nrhs, const double* A, long lda, const int* perm,
+                       const double* R, long ldr, double* Out, long ldo, const GetrsPrep& g);
 
 // Mixed precision LU of the n x n matrix A (read only): the trailing matrix
 // in FP32 (FP32 matrix cores), each panel widened to FP64 and factored by the

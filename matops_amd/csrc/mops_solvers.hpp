@@ -40,6 +40,26 @@ int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double
 int trsm_pair_d(hipStream_t s, int flags1, int flags2, int N, int nrhs, const double* A,
                 long lda, double* B, long ldb);
 
+// LEFT solves repeated with one factor (the refinement's solves): the
+// inverses of the 64x64 diagonal blocks are computed once by trsm_prep_d,
+// trsm_prep_solve_d then solves B = op(A)^-1 B (alpha = 1) with them. With
+// many right-hand sides (no chained solve) nothing is prepared and the
+// solve is trsm_d. W, when not null, is the sentinel-filled copy of the
+// solution the chained solve needs (trsm_prep_w_bytes; may be shared by
+// preparations used one after the other on one stream).
+struct TrsmPrep {
+  double* D = nullptr;
+  double* W = nullptr;
+  long ldw = 0;
+  int nb64 = 0, ngrp = 0;
+};
+size_t trsm_prep_ws_bytes(int N, int nrhs);  // D, W and the inversion's scratch
+size_t trsm_prep_w_bytes(int N, int nrhs);
+int trsm_prep_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda, double* W,
+                TrsmPrep& p);
+int trsm_prep_solve_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda,
+                      double* B, long ldb, const TrsmPrep& p);
+
 // The other BLAS-3 operations (blas3x.hip). symm_d: C (M x N) = beta C +
 // alpha S B (left: S = sym(A), M x M) or alpha B S (S N x N), sym(A) the
 // symmetric matrix whose lower (else upper) triangle A stores; needs

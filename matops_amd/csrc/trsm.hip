@@ -553,6 +553,52 @@ int trsm_pair_d(hipStream_t s, int flags1, int flags2, int N, int nrhs, const do
   return st;
 }
 
+static bool trsm_chained(int flags, int N, int nrhs) {
+  return !(flags & F_RIGHT) && nrhs > 0 && nrhs <= SKINNY_MAX_RHS && N > TC_BS;
+}
+
+size_t trsm_prep_w_bytes(int N, int nrhs) {
+  return (size_t)ceil_div(N, TC_BS) * TC_BS * ceil_div(nrhs, TC_CG) * TC_CG * sizeof(double) + 256;
+}
+
+size_t trsm_prep_ws_bytes(int N, int nrhs) {
+  const int nb64 = ceil_div(N, TC_BS);
+  return (size_t)nb64 * TC_BS * TC_BS * sizeof(double) + trsm_prep_w_bytes(N, nrhs) +
+         trtri_core_ws(TC_BS, nb64) + 3 * 256;
+}
+
+int trsm_prep_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda, double* W,
+                TrsmPrep& p) {
+  p = TrsmPrep();
+  if (!trsm_chained(flags, N, nrhs)) return ST_OK;
+  p.nb64 = ceil_div(N, TC_BS);
+  p.ngrp = ceil_div(nrhs, TC_CG);
+  p.ldw = (long)p.nb64 * TC_BS;
+  p.D = (double*)ws_carve((size_t)p.nb64 * TC_BS * TC_BS * sizeof(double));
+  p.W = W ? W : (double*)ws_carve((size_t)p.ldw * p.ngrp * TC_CG * sizeof(double));
+  if (!p.D || !p.W) {
+    set_error("trsm: workspace too small");
+    return ST_NOMEM;
+  }
+  const size_t mark = ws_mark();  // the inversion's scratch only
+  const int st = trtri_blocks_d(s, !(flags & F_UPPER), (flags & F_UNIT) != 0, N, TC_BS, A, lda,
+                                p.D);
+  ws_release(mark);
+  return st;
+}
+
+int trsm_prep_solve_d(hipStream_t s, int flags, int N, int nrhs, const double* A, long lda,
+                      double* B, long ldb, const TrsmPrep& p) {
+  if (!p.D) return trsm_d(s, flags, N, nrhs, 1.0, A, lda, B, ldb);
+  ChainWs w;
+  w.D = p.D;
+  w.W = p.W;
+  w.ldw = p.ldw;
+  w.nb64 = p.nb64;
+  w.ngrp = p.ngrp;
+  return chain_solve(s, flags, N, nrhs, A, lda, B, ldb, w);
+}
+
 int trsm_d(hipStream_t s, int flags, int N, int nrhs, double alpha, const double* A, long lda,
            double* B, long ldb) {
   const size_t mark = ws_mark();
