@@ -338,8 +338,32 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
     }
     return v;
   };
+  // offsets of this thread's pairs at k = 0 (for full slabs: the slab's
+  // start then adds one uniform term instead of a 64-bit product per pair)
+  long offA[A_PP], offB[B_PP];
+#pragma unroll
+  for (int e = 0; e < A_PP; ++e) {
+    int i, kk;
+    coords(e, BM, !TA, i, kk);
+    offA[e] = !TA ? (m0 + i) + (long)kk * lda : kk + (long)(m0 + i) * lda;
+  }
+#pragma unroll
+  for (int e = 0; e < B_PP; ++e) {
+    int j, kk;
+    coords(e, BN, TB, j, kk);
+    offB[e] = TB ? (n0 + j) + (long)kk * ldb : kk + (long)(n0 + j) * ldb;
+  }
   auto load_to = [&](int k0, pair_t(&xa)[A_PP], pair_t(&xb)[B_PP]) {
     const bool inner = AL || (inner_mn && k0 + BK <= K);
+    if (inner) {
+      const T* Ak = A + (!TA ? (long)k0 * lda : (long)k0);
+      const T* Bk = B + (TB ? (long)k0 * ldb : (long)k0);
+#pragma unroll
+      for (int e = 0; e < A_PP; ++e) xa[e] = *(const pair_t*)(Ak + offA[e]);
+#pragma unroll
+      for (int e = 0; e < B_PP; ++e) xb[e] = *(const pair_t*)(Bk + offB[e]);
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < A_PP; ++e) {
       int i, kk;
