@@ -158,7 +158,13 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void k_gemm(GemmArgs<T> g) {
   // flips bits 2-4 of the column, so element pairs stay contiguous, a row
   // keeps its footprint, and the MFMA reads (16 consecutive columns of rows
   // 4r..4r+3; rows 4r and 4r+1 share the swizzle) keep their bank halves.
-  auto swz = [](int kk, int i) { return i ^ (4 * ((kk >> 1) & 7)); };
+  // When neither operand is staged as k pairs (A not transposed, B
+  // transposed: the Cholesky's SYRK and block-column updates) nothing is
+  // swizzled, so the MFMA reads of a slab share base registers and use
+  // immediate offsets (SYRK n = 7680, k = 512: 46.4 -> 47.9 TFLOP/s).
+  // Dropping it for one operand of the other forms made the LU slower.
+  constexpr bool SW = TA || !TB;
+  auto swz = [](int kk, int i) { return SW ? i ^ (4 * ((kk >> 1) & 7)) : i; };
   auto posA = [&](int kk, int i) { return kk * LDS_A + swz(kk, i); };
   auto posB = [&](int kk, int i) { return kk * LDS_B + swz(kk, i); };
   // operands move in pairs of consecutive elements along their contiguous
