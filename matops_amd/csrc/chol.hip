@@ -259,9 +259,14 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
           dacc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[(16 * tj + li) * CL_LD + s4 + lq], xa,
                                                           dacc[tj], 0, 0, 0);
       }
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (t == 0) __hip_atomic_store(&xflag[CP_MAXB * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // the flag of the last step is published after the first column
+      // block of the factorization, by when the stores of X have drained
+      // without holding up the factorization
+      if (j + 1 < jend) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(&xflag[CP_MAXB * j + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     // updates of the other column blocks right of j, right to left, so that
     // the block of column j + 1 (the next right-hand side) comes last
@@ -393,7 +398,11 @@ __global__ __launch_bounds__(CL_T) void k_chol_panel(int m, int pw, double* P, l
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+    const bool pub = jb == 0 && jend > 0;  // the last step's rows of X (see above)
+    if (pub) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    if (pub && t == 0)
+      __hip_atomic_store(&xflag[CP_MAXB * (jend - 1) + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (w == jb) {
       // the finished columns back to A (off the serial path)
 #pragma unroll
